@@ -1,0 +1,51 @@
+"""Build libadfvm.so in-tree: hipcc for the gfx950 kernels, g++ for the
+host mesh builder (strict FP evaluation order), linked into one shared
+library next to this file.
+
+    python -m adfvm_amd.build
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, 'csrc')
+INCLUDE = os.path.join(ROOT, 'include')
+BUILD_DIR = os.path.join(ROOT, '_build')
+LIB = os.path.join(HERE, 'libadfvm.so')
+ARCH = 'gfx950'
+
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+# packed FP32 instructions are switched off for all device code
+HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=' + ARCH,
+             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+             '-munsafe-fp-atomics', '-I', INCLUDE, '-I', CSRC]
+CXX_FLAGS = ['-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', '-I', INCLUDE]
+
+
+def run(cmd):
+    print(' '.join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    objs = []
+    mesh_obj = os.path.join(BUILD_DIR, 'mesh_build.o')
+    run(['g++'] + CXX_FLAGS + ['-c', os.path.join(CSRC, 'mesh_build.cpp'), '-o', mesh_obj])
+    objs.append(mesh_obj)
+    for src in sorted(os.listdir(CSRC)):
+        if src.endswith('.hip'):
+            obj = os.path.join(BUILD_DIR, src[:-4] + '.o')
+            run([HIPCC] + HIP_FLAGS + ['-c', os.path.join(CSRC, src), '-o', obj])
+            objs.append(obj)
+    tmp = LIB + '.tmp'
+    run([HIPCC, '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == '__main__':
+    build()
+    sys.exit(0)

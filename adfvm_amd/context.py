@@ -1,0 +1,245 @@
+"""Device context: one libadfvm context per (mesh, physics, boundary set).
+
+Replaces adpy's `Function.initialize(localRank, mesh)` -> `external_init`
+(adFVM/solver.py:70, adFVM/cpp/external.cpp:21-42): the mesh and boundary
+values are uploaded once; each step then passes only device state pointers.
+
+Tensors passed in are torch tensors on the current HIP device, contiguous,
+in the context precision; state tensors have shape (5, nInternalCells).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+BC_CODES = {
+    'cyclic': _lib.BC_CYCLIC, 'slidingPeriodic1D': _lib.BC_CYCLIC,
+    'zeroGradient': _lib.BC_ZERO_GRADIENT, 'empty': _lib.BC_ZERO_GRADIENT, 'inletOutlet': _lib.BC_ZERO_GRADIENT,
+    'symmetryPlane': _lib.BC_SYMMETRY, 'slip': _lib.BC_SYMMETRY,
+    'fixedValue': _lib.BC_FIXED_VALUE, 'calculated': _lib.BC_CALCULATED,
+    # characteristic conditions on U/T leave the field alone; p carries the rule
+    'CBC_UPT': _lib.BC_CALCULATED, 'CBC_TOTAL_PT': _lib.BC_CALCULATED,
+}
+
+
+def _need_torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError('adfvm_amd kernels need a HIP device (torch.cuda.is_available() is False)')
+    return torch
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(None)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(stream)
+
+
+class Context(object):
+    """Owns an adfvm_ctx.
+
+    mesh: adfvm_amd.mesh.Mesh (built).
+    patch_bcs: dict patch -> {'U': spec, 'T': spec, 'p': spec} with spec a dict
+      {'type': ..., 'value': array} in adFVM's boundaryField form; missing
+      entries use the mesh's default boundary (mesh.py:702-711).
+    physics: dict(gamma, Cp, Pr, mu=(model, mu0, muA, muTs), riemann,
+      boundaryRiemann, small, integrator, objective, objectivePatch).
+    """
+
+    def __init__(self, mesh, patch_bcs, physics, precision):
+        self.torch = _need_torch()
+        lib = _lib.load()
+        self.lib = lib
+        self.mesh = mesh
+        self.precision = np.dtype(precision)
+        self.nIC = mesh.nInternalCells
+        self.nBF = mesh.nFaces - mesh.nInternalFaces
+        self._keep = []
+        names = list(mesh.boundary.keys())
+        descs = (_lib.PatchDesc * max(1, len(names)))()
+        for i, name in enumerate(names):
+            b = mesh.boundary[name]
+            spec = patch_bcs.get(name, {})
+            default = mesh.defaultBoundary[name]['type']
+            fspec = {f: spec.get(f, {'type': default}) for f in ('U', 'T', 'p')}
+            ptype = b['type']
+            cbc = _lib.CBC_NONE
+            if fspec['p']['type'] == 'CBC_UPT':
+                cbc = _lib.CBC_UPT
+            elif fspec['p']['type'] == 'CBC_TOTAL_PT':
+                cbc = _lib.CBC_TOTAL_PT
+            if ptype in ('cyclic', 'slidingPeriodic1D'):
+                flux = _lib.FLUX_COUPLED
+            elif cbc != _lib.CBC_NONE:
+                flux = _lib.FLUX_CHARACTERISTIC
+            else:
+                flux = _lib.FLUX_BOUNDARY
+            n = b['nFaces']
+            d = descs[i]
+            d.startFace, d.nFaces, d.fluxKind = b['startFace'], n, flux
+            d.neighbourPatch = names.index(b['neighbourPatch']) if flux == _lib.FLUX_COUPLED else -1
+            d.bcU, d.bcT, d.bcp = [BC_CODES[fspec[f]['type']] for f in ('U', 'T', 'p')]
+            d.cbc = cbc
+
+            def arr(v, k):
+                if v is None:
+                    return None
+                a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64).reshape(-1, k) if np.size(v) != k
+                                                         else np.asarray(v, np.float64).reshape(1, k), (n, k)))
+                self._keep.append(a)
+                return a
+            if cbc == _lib.CBC_UPT:
+                ps = fspec['p']
+                vU, vT, vp = arr(ps['U0'], 3), arr(ps['T0'], 1), arr(ps['p0'], 1)
+            else:
+                vU = arr(fspec['U'].get('value'), 3)
+                vT = arr(fspec['T'].get('value'), 1)
+                vp = arr(fspec['p'].get('value'), 1)
+            d.valueU, d.valueT, d.valuep = [_lib.ptr(a, C.c_double) for a in (vU, vT, vp)]
+            if cbc == _lib.CBC_TOTAL_PT:
+                ps = fspec['p']
+                d.Tt = _lib.ptr(arr(ps['Tt'], 1), C.c_double)
+                d.pt = _lib.ptr(arr(ps['pt'], 1), C.c_double)
+                d.direction = _lib.ptr(arr(ps.get('direction'), 3), C.c_double)
+        self._descs = descs
+
+        sz = _lib.MeshSizes(mesh.nInternalFaces, mesh.nFaces, self.nBF, mesh.nInternalCells,
+                            mesh.nGhostCells, mesh.nCells)
+        c64 = lambda a: _lib.ptr(np.ascontiguousarray(a, np.float64), C.c_double)
+        c32 = lambda a: _lib.ptr(np.ascontiguousarray(a, np.int32), C.c_int32)
+        hold = [np.ascontiguousarray(getattr(mesh, k)) for k in
+                ('owner', 'neighbour', 'cellFaces', 'cellOwner', 'cellNeighbours', 'areas', 'normals',
+                 'weights', 'deltas', 'deltasUnit', 'linearWeights', 'quadraticWeights', 'volumes')]
+        desc = _lib.SolverDesc(sz, c32(hold[0]), c32(hold[1]), c32(hold[2]), c32(hold[3]), c32(hold[4]),
+                               c64(hold[5]), c64(hold[6]), c64(hold[7]), c64(hold[8]), c64(hold[9]),
+                               c64(hold[10]), c64(hold[11]), c64(hold[12]), len(names), descs)
+        ph = _lib.Physics()
+        ph.precision = self.precision.itemsize
+        ph.gamma, ph.Cp, ph.Pr = physics['gamma'], physics['Cp'], physics['Pr']
+        ph.muModel, ph.mu0, ph.muA, ph.muTs = physics['mu']
+        ph.riemann = physics.get('riemann', _lib.RIEMANN_ROE)
+        ph.boundaryRiemann = physics.get('boundaryRiemann', _lib.RIEMANN_ROE)
+        ph.small = physics['small']
+        ph.integrator = physics.get('integrator', _lib.RK_SSPRK3)
+        ph.objective = physics.get('objective', _lib.OBJ_NONE)
+        obj_patch = physics.get('objectivePatch')
+        ph.objectivePatch = names.index(obj_patch) if obj_patch is not None else -1
+        ph.limiter = int(physics.get('limiter', 0))
+        handle = C.c_void_p()
+        rc = lib.adfvm_ctx_create(C.byref(desc), C.byref(ph), C.byref(handle))
+        self.handle = handle
+        if rc != _lib.ADFVM_OK:
+            msg = lib.adfvm_last_error(handle).decode() if handle.value else ''
+            lib.adfvm_ctx_destroy(handle)
+            self.handle = None
+            raise _lib.LibraryError('adfvm_ctx_create failed (%d): %s' % (rc, msg))
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h is not None and h.value:
+            self.lib.adfvm_ctx_destroy(h)
+            self.handle = None
+
+    # ------------------------------------------------------------ helpers --
+    @property
+    def tdtype(self):
+        return self.torch.float32 if self.precision == np.float32 else self.torch.float64
+
+    def _chk(self, rc):
+        _lib.check(rc, self.handle)
+
+    def _check(self, t, shape):
+        assert t.is_cuda and t.is_contiguous() and t.dtype == self.tdtype, (t.device, t.dtype)
+        assert tuple(t.shape) == tuple(shape), (tuple(t.shape), shape)
+
+    def state(self, fill=None):
+        t = self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
+        if fill is not None:
+            t.fill_(fill)
+        return t
+
+    # ------------------------------------------------------------ entries --
+    def primal_step(self, u, dt, source=None, dtc_out=None, obj_out=None, stream=None):
+        self._check(u, (5, self.nIC))
+        if source is not None:
+            self._check(source, (5, self.nIC))
+        self._chk(self.lib.adfvm_primal_step(self.handle, _dptr(u), float(dt), _dptr(source), _dptr(dtc_out),
+                                             _dptr(obj_out), _stream(stream)))
+
+    def adjoint_step(self, u_n, dt, ua, obja=1.0, source=None, source_grad=None, stream=None):
+        self._check(u_n, (5, self.nIC))
+        self._check(ua, (5, self.nIC))
+        self._chk(self.lib.adfvm_adjoint_step(self.handle, _dptr(u_n), float(dt), _dptr(source), _dptr(ua),
+                                              float(obja), _dptr(source_grad), _stream(stream)))
+
+    def gradients(self, u, out=None, stream=None):
+        self._check(u, (5, self.nIC))
+        out = out if out is not None else self.torch.empty((15, self.nIC), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_gradients(self.handle, _dptr(u), _dptr(out), _stream(stream)))
+        return out
+
+    def residual(self, u, out=None, dtc_cell=None, stream=None):
+        self._check(u, (5, self.nIC))
+        out = out if out is not None else self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_residual(self.handle, _dptr(u), _dptr(out), _dptr(dtc_cell), _stream(stream)))
+        return out
+
+    def residual_vjp(self, u, rbar, out=None, stream=None):
+        out = out if out is not None else self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_residual_vjp(self.handle, _dptr(u), _dptr(rbar), _dptr(out), _stream(stream)))
+        return out
+
+    def objective(self, u, out=None, stream=None):
+        out = out if out is not None else self.torch.zeros(1, dtype=self.torch.float64, device='cuda')
+        self._chk(self.lib.adfvm_objective(self.handle, _dptr(u), _dptr(out), _stream(stream)))
+        return out
+
+    def primitive_ghosts(self, u, out=None, stream=None):
+        nC = self.mesh.nCells
+        out = out if out is not None else self.torch.empty((5, nC), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_primitive_ghosts(self.handle, _dptr(u), _dptr(out), _stream(stream)))
+        return out
+
+    def grad_cell(self, phi, stream=None):
+        k = phi.shape[0]
+        out = self.torch.empty((3 * k, self.nIC), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_grad_cell(self.handle, _dptr(phi), k, _dptr(out), _stream(stream)))
+        return out
+
+    def interp_second_order(self, phi, grad, swap, stream=None):
+        k = phi.shape[0]
+        out = self.torch.empty((k, self.mesh.nFaces), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_interp_second_order(self.handle, _dptr(phi), _dptr(grad), k, swap, _dptr(out),
+                                                     _stream(stream)))
+        return out
+
+    def dot(self, a, b, out=None, stream=None):
+        out = out if out is not None else self.torch.zeros(1, dtype=self.torch.float64, device='cuda')
+        self._chk(self.lib.adfvm_dot(self.handle, _dptr(a), _dptr(b), a.numel(), _dptr(out), _stream(stream)))
+        return out
+
+    def adjoint_energy(self, ua, u, out=None, stream=None):
+        out = out if out is not None else self.torch.zeros(1, dtype=self.torch.float64, device='cuda')
+        self._chk(self.lib.adfvm_adjoint_energy(self.handle, _dptr(ua), _dptr(u), _dptr(out), _stream(stream)))
+        return out
+
+    def pack(self, rho, rhoU, rhoE, out=None, stream=None):
+        out = out if out is not None else self.state()
+        self._chk(self.lib.adfvm_pack_state(self.handle, _dptr(rho), _dptr(rhoU), _dptr(rhoE), _dptr(out),
+                                            _stream(stream)))
+        return out
+
+    def unpack(self, u, stream=None):
+        t = self.torch
+        rho = t.empty((self.nIC, 1), dtype=self.tdtype, device='cuda')
+        rhoU = t.empty((self.nIC, 3), dtype=self.tdtype, device='cuda')
+        rhoE = t.empty((self.nIC, 1), dtype=self.tdtype, device='cuda')
+        self._chk(self.lib.adfvm_unpack_state(self.handle, _dptr(u), _dptr(rho), _dptr(rhoU), _dptr(rhoE),
+                                              _stream(stream)))
+        return rho, rhoU, rhoE

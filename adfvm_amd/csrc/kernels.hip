@@ -1,0 +1,916 @@
+// libadfvm: HIP kernels for gfx950 (MI355X) and the C ABI of include/adfvm.h.
+//
+// Design (one GPU, one context per mesh):
+//  * state and gradients are structure-of-arrays rows of nInternalCells
+//    scalars in the context precision (float or double);
+//  * every face->cell accumulation is a gather: each cell walks its six
+//    cellFaces slots in order (owned faces in face order, then neighbour
+//    faces, cmesh.cpp:9-19) and computes the face quantities it needs.  No
+//    atomics: results are bit-identical from run to run;
+//  * cyclic ghosts are resolved at set-up to the paired internal cell, so a
+//    periodic mesh needs no ghost storage; non-cyclic boundary faces get a
+//    small ghost array of primitive values refreshed once per stage;
+//  * reductions (objective, dtc) write one partial per block and a single
+//    block folds the partials in index order, in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "adfvm.h"
+#include "fvm_math.hpp"
+#include "adjoint_math.hpp"
+
+using namespace adfvm;
+
+#define BLOCK 256
+
+// ------------------------------------------------------------- tables ----
+// slot flags: bit 0 = this cell owns the face; bits 1-2 = slot kind
+enum { SLOT_INTERNAL = 0, SLOT_COUPLED = 1, SLOT_CHARACTERISTIC = 2, SLOT_BOUNDARY = 3 };
+
+// face geometry rows (each nFaces long)
+enum { G_A = 0, G_NX, G_NY, G_NZ, G_W, G_DELTA, G_DX, G_DY, G_DZ, G_LW0, G_LW1,
+       G_Q0X, G_Q0Y, G_Q0Z, G_Q1X, G_Q1Y, G_Q1Z, G_ROWS };
+
+struct DPatch {
+    int start, n, fluxKind, bcU, bcT, bcp, cbc, hasDir;
+    long valU, valT, valp, valTt, valpt, valDir;   // offsets into the value buffer, -1 if absent
+};
+
+template <typename T> struct DMesh {
+    int nIC, nIF, nF, nBF;
+    const T* geo;           // [G_ROWS][nF]
+    const T* vol;           // [nIC]
+    const int* sFace;       // [6][nIC]
+    const int* sOther;      // [6][nIC]: internal cell, or -1-b for boundary face b
+    const unsigned char* sFlags;  // [6][nIC]
+    const int* sPair;       // [6][nIC]: paired face of a coupled slot, else -1
+    const int* owner;       // [nF]
+    const int* bPatch;      // [nBF]
+    const DPatch* patches;
+    const T* bcval;
+};
+
+template <typename T> __device__ __forceinline__ void load_geo(const DMesh<T>& m, int f, FaceGeo<T>& g) {
+    const T* G = m.geo;
+    const long s = m.nF;
+    g.A = G[G_A * s + f];
+    g.n[0] = G[G_NX * s + f]; g.n[1] = G[G_NY * s + f]; g.n[2] = G[G_NZ * s + f];
+    g.w = G[G_W * s + f];
+    g.delta = G[G_DELTA * s + f];
+    g.D[0] = G[G_DX * s + f]; g.D[1] = G[G_DY * s + f]; g.D[2] = G[G_DZ * s + f];
+    g.lw0 = G[G_LW0 * s + f]; g.lw1 = G[G_LW1 * s + f];
+    g.qw0[0] = G[G_Q0X * s + f]; g.qw0[1] = G[G_Q0Y * s + f]; g.qw0[2] = G[G_Q0Z * s + f];
+    g.qw1[0] = G[G_Q1X * s + f]; g.qw1[1] = G[G_Q1Y * s + f]; g.qw1[2] = G[G_Q1Z * s + f];
+}
+
+// primitive values of a value source: internal cell (from the state) or ghost
+template <typename T>
+__device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict__ u, long nIC,
+                                          const T* __restrict__ ghost, long nBF, int src, T* w) {
+    if (src >= 0) {
+        T q[5];
+        for (int i = 0; i < 5; i++) q[i] = u[i * nIC + src];
+        primitive(ph, q, w);
+    } else {
+        int b = -1 - src;
+        for (int i = 0; i < 5; i++) w[i] = ghost[i * nBF + b];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_grad(const T* __restrict__ grad, long nIC, int c, T* g) {
+    for (int i = 0; i < 15; i++) g[i] = grad[i * nIC + c];
+}
+
+// ------------------------------------------------------------- kernels ---
+
+// Ghost primitive values of non-cyclic boundary faces (Solver.boundary +
+// characteristicBoundary, solver.py:106-114, density.py:333-337, BCs.py).
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) ghost_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+                                                      T* __restrict__ ghost) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nBF) return;
+    const DPatch P = m.patches[m.bPatch[b]];
+    if (P.bcU == ADFVM_BC_CYCLIC && P.cbc == ADFVM_CBC_NONE) return;   // resolved by the slot tables
+    const int f = m.nIF + b;
+    const int k = f - P.start;
+    const long nIC = m.nIC, nBF = m.nBF;
+    const int c = m.owner[f];
+    T q[5], w[5], out[5];
+    for (int i = 0; i < 5; i++) q[i] = u[i * nIC + c];
+    primitive(ph, q, w);
+    const T* V = m.bcval;
+    if (P.cbc == ADFVM_CBC_UPT) {                    // BCs.py:152-161
+        for (int i = 0; i < 3; i++) out[i] = V[P.valU + 3 * k + i];
+        out[3] = V[P.valT + k];
+        out[4] = V[P.valp + k];
+    } else if (P.cbc == ADFVM_CBC_TOTAL_PT) {        // BCs.py:164-184
+        T d[3];
+        if (P.hasDir) { for (int i = 0; i < 3; i++) d[i] = V[P.valDir + 3 * k + i]; }
+        else { const long s = m.nF; d[0] = m.geo[G_NX * s + f]; d[1] = m.geo[G_NY * s + f]; d[2] = m.geo[G_NZ * s + f]; }
+        T Tt = V[P.valTt + k], pt = V[P.valpt + k];
+        T Un = w[0] * d[0] + w[1] * d[1] + w[2] * d[2];
+        for (int i = 0; i < 3; i++) out[i] = Un * d[i];
+        out[3] = Tt - T(0.5) * Un * Un / ph.Cp;
+        out[4] = pt * pow(w[3] / Tt, ph.gamma / (ph.gamma - T(1)));
+    } else {
+        // U
+        if (P.bcU == ADFVM_BC_FIXED_VALUE || P.bcU == ADFVM_BC_CALCULATED) {
+            for (int i = 0; i < 3; i++) out[i] = V[P.valU + 3 * k + i];
+        } else {
+            for (int i = 0; i < 3; i++) out[i] = w[i];
+            if (P.bcU == ADFVM_BC_SYMMETRY) {        // BCs.py:125-133
+                const long s = m.nF;
+                T n0 = m.geo[G_NX * s + f], n1 = m.geo[G_NY * s + f], n2 = m.geo[G_NZ * s + f];
+                T un = out[0] * n0 + out[1] * n1 + out[2] * n2;
+                out[0] -= un * n0; out[1] -= un * n1; out[2] -= un * n2;
+            }
+        }
+        out[3] = (P.bcT == ADFVM_BC_FIXED_VALUE || P.bcT == ADFVM_BC_CALCULATED) ? V[P.valT + k] : w[3];
+        out[4] = (P.bcp == ADFVM_BC_FIXED_VALUE || P.bcp == ADFVM_BC_CALCULATED) ? V[P.valp + k] : w[4];
+    }
+    for (int i = 0; i < 5; i++) ghost[i * nBF + b] = out[i];
+}
+
+// Gauss cell gradients of U, T, p (op.gradCell, op.py:45-63).
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) grad_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+                                                     const T* __restrict__ ghost, T* __restrict__ grad) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nIC) return;
+    const long nIC = m.nIC, nF = m.nF;
+    T wc[5];
+    load_prim(ph, u, nIC, ghost, m.nBF, c, wc);
+    T g[15];
+    for (int i = 0; i < 15; i++) g[i] = T(0);
+    for (int j = 0; j < 6; j++) {
+        const int f = m.sFace[j * nIC + c];
+        const int o = m.sOther[j * nIC + c];
+        const bool own = m.sFlags[j * nIC + c] & 1;
+        T wo[5];
+        load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
+        T S = m.geo[G_A * nF + f];
+        T N[3] = {m.geo[G_NX * nF + f], m.geo[G_NY * nF + f], m.geo[G_NZ * nF + f]};
+        T w = m.geo[G_W * nF + f];
+        if (!own) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
+        else w = T(1) - w;
+        T SN[3] = {S * N[0], S * N[1], S * N[2]};
+        for (int i = 0; i < 5; i++) {
+            T phiF = wc[i] * (T(1) - w) + wo[i] * w;
+            g[3 * i + 0] += phiF * SN[0];
+            g[3 * i + 1] += phiF * SN[1];
+            g[3 * i + 2] += phiF * SN[2];
+        }
+    }
+    T V = m.vol[c];
+    for (int i = 0; i < 15; i++) grad[i * nIC + c] = g[i] / V;
+}
+
+// Block reduction of one double per thread into out[blockIdx.x]; op 0 sum, 1 max.
+template <int OP>
+__device__ void block_reduce_store(double v, double* out) {
+    __shared__ double sh[BLOCK];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double a = sh[threadIdx.x], b = sh[threadIdx.x + s];
+            sh[threadIdx.x] = OP == 0 ? a + b : (a > b ? a : b);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// Folds n partials in index order inside one block: out = op(partials).
+template <int OP>
+__global__ void __launch_bounds__(BLOCK) finalize_kernel(const double* __restrict__ partials, int n, double* out) {
+    double v = OP == 0 ? 0.0 : -1.0e308;
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        double x = partials[i];
+        v = OP == 0 ? v + x : (v > x ? v : x);
+    }
+    __shared__ double sh[BLOCK];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double a = sh[threadIdx.x], b = sh[threadIdx.x + s];
+            sh[threadIdx.x] = OP == 0 ? a + b : (a > b ? a : b);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// Residual and stage update, one cell per thread:
+//   L_c = sum_slots sign*F_f*(A_f/V_c)   (op.div, op.py:12-20)
+//   lam_c = sum_slots lam_f*(A_f/V_c)    (op.absDiv, op.py:22-29)
+// MODE 0: uout = a0*u0 + ai*ui + (-beta*(L-S))*dt   (timestep.update, timestep.py:35-45)
+// MODE 1: uout = L (the residual)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(BLOCK) flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ ui,
+                                                     const T* __restrict__ grad, const T* __restrict__ ghost,
+                                                     const T* __restrict__ u0, T a0, T ai, T nbeta, T dt,
+                                                     const T* __restrict__ S, T* __restrict__ uout,
+                                                     T* __restrict__ dtcCell, double* __restrict__ dtcPartial) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const long nIC = m.nIC, nBF = m.nBF;
+    double dtcv = 0.0;
+    if (c < m.nIC) {
+        T wc[5], gc[15];
+        load_prim(ph, ui, nIC, ghost, nBF, c, wc);
+        load_grad(grad, nIC, c, gc);
+        const T invV = T(1);
+        T L[5] = {0, 0, 0, 0, 0};
+        T lamSum = 0;
+        const T V = m.vol[c];
+        (void)invV;
+        for (int j = 0; j < 6; j++) {
+            const int f = m.sFace[j * nIC + c];
+            const int o = m.sOther[j * nIC + c];
+            const int fl = m.sFlags[j * nIC + c];
+            const bool own = fl & 1;
+            const int kind = fl >> 1;
+            FaceGeo<T> fg;
+            load_geo(m, f, fg);
+            T wo[5], go[15], F[5], lam;
+            load_prim(ph, ui, nIC, ghost, nBF, o, wo);
+            if (kind == SLOT_INTERNAL || kind == SLOT_COUPLED) {
+                load_grad(grad, nIC, o, go);
+                if (own) face_flux(ph, FACE_INTERIOR, wc, gc, wo, go, fg, F, &lam);
+                else face_flux(ph, FACE_INTERIOR, wo, go, wc, gc, fg, F, &lam);
+            } else {
+                face_flux(ph, kind == SLOT_CHARACTERISTIC ? FACE_CHARACTERISTIC : FACE_BOUNDARY, wc, gc, wo, gc, fg, F, &lam);
+            }
+            T wgt = fg.A / V;
+            T sw = own ? wgt : -wgt;
+            for (int i = 0; i < 5; i++) L[i] += F[i] * sw;
+            lamSum += lam * wgt;
+        }
+        if (MODE == 1) {
+            for (int i = 0; i < 5; i++) uout[i * nIC + c] = L[i];
+        } else {
+            for (int i = 0; i < 5; i++) {
+                T acc = ai * ui[i * nIC + c];
+                if (u0) acc = a0 * u0[i * nIC + c] + acc;
+                T Li = S ? L[i] - S[i * nIC + c] : L[i];
+                uout[i * nIC + c] = acc + (nbeta * Li) * dt;
+            }
+        }
+        if (dtcCell) dtcCell[c] = lamSum;
+        dtcv = (double)lamSum;
+    }
+    if (dtcPartial) block_reduce_store<1>(dtcv, dtcPartial);
+}
+
+// Objective partial sums (FP64): kinetic energy 0.5*rho*|U|^2*V per cell.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) ke_kernel(DMesh<T> m, const T* __restrict__ u, double* partial) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const long nIC = m.nIC;
+    double v = 0.0;
+    if (c < m.nIC) {
+        T rho = u[c];
+        T Ux = u[nIC + c] / rho, Uy = u[2 * nIC + c] / rho, Uz = u[3 * nIC + c] / rho;
+        T ke = T(0.5) * rho * (Ux * Ux + Uy * Uy + Uz * Uz) * m.vol[c];
+        v = (double)ke;
+    }
+    block_reduce_store<0>(v, partial);
+}
+
+// Drag on one patch (templates/cylinder_test.py:9-19): sum (p0 nx - mu(T0)(U0x-Uix)/delta) A.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) drag_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+                                                     const T* __restrict__ ghost, int start, int n, double* partial) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (k < n) {
+        const int f = start + k;
+        const int b = f - m.nIF;
+        const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
+        T wi[5];
+        load_prim(ph, u, nIC, ghost, nBF, m.owner[f], wi);
+        T U0 = ghost[b], p0 = ghost[4 * nBF + b], T0 = ghost[3 * nBF + b];
+        T nx = m.geo[G_NX * nF + f];
+        T mungUx = viscosity(ph, T0) * (U0 - wi[0]) / m.geo[G_DELTA * nF + f];
+        v = (double)((p0 * nx - mungUx) * m.geo[G_A * nF + f]);
+    }
+    block_reduce_store<0>(v, partial);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) prim_ghost_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+                                                           const T* __restrict__ ghost, T* __restrict__ prim) {
+    // prim [5][nC]: internal rows from the state, ghost rows from the ghost
+    // array or (cyclic) the paired owner.
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long nIC = m.nIC, nBF = m.nBF, nC = (long)m.nIC + m.nBF;
+    if (i >= nC) return;
+    T w[5];
+    if (i < m.nIC) {
+        load_prim(ph, u, nIC, ghost, nBF, i, w);
+    } else {
+        int b = i - m.nIC;
+        const DPatch P = m.patches[m.bPatch[b]];
+        if (P.bcU == ADFVM_BC_CYCLIC && P.cbc == ADFVM_CBC_NONE) {
+            // cyclic ghost: the paired owner, as resolved in the owner's slot
+            int f = m.nIF + b;
+            int c = m.owner[f];
+            int partner = -1;
+            for (int j = 0; j < 6; j++)
+                if (m.sFace[j * nIC + c] == f) partner = m.sOther[j * nIC + c];
+            load_prim(ph, u, nIC, ghost, nBF, partner, w);
+        } else {
+            for (int r = 0; r < 5; r++) w[r] = ghost[r * nBF + b];
+        }
+    }
+    for (int r = 0; r < 5; r++) prim[r * nC + i] = w[r];
+}
+
+// op.gradCell over a caller field given on all nCells (reference cellNeighbours,
+// no boundary update), k components.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) grad_field_kernel(DMesh<T> m, const int* __restrict__ cellNb,
+                                                           const T* __restrict__ phi, int k, long nC,
+                                                           T* __restrict__ grad) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nIC) return;
+    const long nIC = m.nIC, nF = m.nF;
+    for (int comp = 0; comp < k; comp++) {
+        T g0 = 0, g1 = 0, g2 = 0;
+        T pc = phi[comp * nC + c];
+        for (int j = 0; j < 6; j++) {
+            const int f = m.sFace[j * nIC + c];
+            const bool own = m.sFlags[j * nIC + c] & 1;
+            const int nb = cellNb[j * nIC + c];
+            T S = m.geo[G_A * nF + f];
+            T N[3] = {m.geo[G_NX * nF + f], m.geo[G_NY * nF + f], m.geo[G_NZ * nF + f]};
+            T w = m.geo[G_W * nF + f];
+            if (!own) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
+            else w = T(1) - w;
+            T phiF = pc * (T(1) - w) + phi[comp * nC + nb] * w;
+            g0 += phiF * (S * N[0]); g1 += phiF * (S * N[1]); g2 += phiF * (S * N[2]);
+        }
+        T V = m.vol[c];
+        grad[(3 * comp + 0) * nIC + c] = g0 / V;
+        grad[(3 * comp + 1) * nIC + c] = g1 / V;
+        grad[(3 * comp + 2) * nIC + c] = g2 / V;
+    }
+}
+
+// interp.secondOrder on every face: phi [k][nC], grad [3k][nC], out [k][nF].
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) interp_kernel(DMesh<T> m, const int* __restrict__ nbFull,
+                                                       const T* __restrict__ phi, const T* __restrict__ grad,
+                                                       int k, long nC, int swap, T* __restrict__ out) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    const long nF = m.nF;
+    int p = m.owner[f], n = nbFull[f];
+    if (swap) { int t = p; p = n; n = t; }
+    T lw = m.geo[(swap ? G_LW1 : G_LW0) * nF + f];
+    T qw[3];
+    for (int i = 0; i < 3; i++) qw[i] = m.geo[((swap ? G_Q1X : G_Q0X) + i) * nF + f];
+    for (int comp = 0; comp < k; comp++) {
+        T pc = phi[comp * nC + p], pd = phi[comp * nC + n];
+        T g0 = grad[(3 * comp) * nC + p], g1 = grad[(3 * comp + 1) * nC + p], g2 = grad[(3 * comp + 2) * nC + p];
+        out[comp * nF + f] = pc + (pd - pc) * lw + (qw[0] * g0 + qw[1] * g1 + qw[2] * g2);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) dot_kernel(const T* __restrict__ a, const T* __restrict__ b, long n,
+                                                    double* partial) {
+    double v = 0.0;
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * BLOCK)
+        v += (double)a[i] * (double)b[i];
+    block_reduce_store<0>(v, partial);
+}
+
+// adjoint energy, cmesh.cpp:342-425 (per-cell entropy-variable quadratic form)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) energy_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ ua,
+                                                       const T* __restrict__ u, double* partial) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long nIC = m.nIC;
+    double e = 0.0;
+    if (i < m.nIC) {
+        double g = (double)ph.gamma;
+        double v = m.vol[i];
+        double rho = u[i], rE = u[4 * nIC + i];
+        double u1 = u[nIC + i] / rho, u2 = u[2 * nIC + i] / rho, u3 = u[3 * nIC + i] / rho;
+        double q2 = u1 * u1 + u2 * u2 + u3 * u3;
+        double p = (rE - rho * q2 / 2) * (g - 1);
+        double H = g * p / (rho * (g - 1)) + q2 / 2;
+        double w1 = ua[i] / v, w2 = ua[nIC + i] / v, w3 = ua[2 * nIC + i] / v, w4 = ua[3 * nIC + i] / v,
+               w5 = ua[4 * nIC + i] / v;
+        double A11 = rho, A12 = rho * u1, A13 = rho * u2, A14 = rho * u3, A15 = rE;
+        double A22 = rho * u1 * u1 + p, A23 = rho * u1 * u2, A24 = rho * u1 * u3, A25 = rho * H * u1;
+        double A33 = rho * u2 * u2 + p, A34 = rho * u2 * u3, A35 = rho * H * u2;
+        double A44 = rho * u3 * u3 + p, A45 = rho * H * u3;
+        double A55 = rho * H * H - g * p * p / (rho * (g - 1));
+        e = (A11 * w1 * w1 + A22 * w2 * w2 + A33 * w3 * w3 + A44 * w4 * w4 + A55 * w5 * w5 +
+             2 * (A12 * w1 * w2 + A13 * w1 * w3 + A14 * w1 * w4 + A15 * w1 * w5 + A23 * w2 * w3 + A24 * w2 * w4 +
+                  A25 * w2 * w5 + A34 * w3 * w4 + A35 * w3 * w5 + A45 * w4 * w5)) * v;
+    }
+    block_reduce_store<0>(e, partial);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) pack_kernel(long n, const T* __restrict__ rho, const T* __restrict__ rhoU,
+                                                     const T* __restrict__ rhoE, T* __restrict__ u) {
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= n) return;
+    u[c] = rho[c];
+    u[n + c] = rhoU[3 * c]; u[2 * n + c] = rhoU[3 * c + 1]; u[3 * n + c] = rhoU[3 * c + 2];
+    u[4 * n + c] = rhoE[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) unpack_kernel(long n, const T* __restrict__ u, T* __restrict__ rho,
+                                                       T* __restrict__ rhoU, T* __restrict__ rhoE) {
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= n) return;
+    rho[c] = u[c];
+    rhoU[3 * c] = u[n + c]; rhoU[3 * c + 1] = u[2 * n + c]; rhoU[3 * c + 2] = u[3 * n + c];
+    rhoE[c] = u[4 * n + c];
+}
+
+__global__ void fill_kernel(double* p, int n, double v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) copy_kernel(long n, const T* __restrict__ x, T* __restrict__ y) {
+    long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) axpby_kernel(long n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
+    long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+#include "adjoint_kernels.inc"
+
+// ----------------------------------------------------------- context -----
+
+struct adfvm_ctx {
+    int prec = 4;
+    int nIC = 0, nIF = 0, nF = 0, nBF = 0, nC = 0;
+    int nPatches = 0;
+    bool hasGhostKernel = false;
+    int objective = ADFVM_OBJ_NONE, objectivePatch = -1;
+    int integrator = ADFVM_RK_SSPRK3;
+    Phys<double> phd;
+    std::string err;
+    std::vector<DPatch> hpatches;
+    // device
+    void* geo = nullptr; void* vol = nullptr; void* bcval = nullptr;
+    int* sFace = nullptr; int* sOther = nullptr; unsigned char* sFlags = nullptr; int* sPair = nullptr;
+    int* owner = nullptr; int* bPatch = nullptr; DPatch* patches = nullptr;
+    int* cellNbFull = nullptr;   // reference cellNeighbours [6][nIC]
+    int* nbFull = nullptr;       // ghost-extended neighbour [nF]
+    void* ghost = nullptr;
+    void* u1 = nullptr; void* u2 = nullptr; void* u3 = nullptr;
+    void* grad0 = nullptr; void* grad1 = nullptr; void* grad2 = nullptr;
+    void* adj0 = nullptr; void* adj1 = nullptr; void* adjAcc = nullptr; void* adjRbar = nullptr;
+    void* adjG = nullptr;      // [15][nIC] gradient adjoint
+    void* adjW = nullptr;      // [5][nIC] primitive adjoint
+    void* adjGhost = nullptr;  // [5][nBF]
+    double* partial = nullptr; int nPartial = 0;
+    double* scal = nullptr;    // small device scalars
+};
+
+namespace {
+
+int set_err(adfvm_ctx* ctx, int code, const std::string& m) {
+    if (ctx) ctx->err = m;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return set_err(ctx, ADFVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int grid_for(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+template <typename T> Phys<T> phys_cast(const Phys<double>& p) {
+    Phys<T> o;
+    o.gamma = (T)p.gamma; o.Cp = (T)p.Cp; o.Cv = (T)p.Cv; o.Pr = (T)p.Pr; o.invCv = (T)p.invCv;
+    o.small = (T)p.small; o.mu0 = (T)p.mu0; o.muA = (T)p.muA; o.muTs = (T)p.muTs;
+    o.muModel = p.muModel; o.riemann = p.riemann; o.boundaryRiemann = p.boundaryRiemann; o.limit = p.limit;
+    return o;
+}
+
+template <typename T> DMesh<T> dmesh(const adfvm_ctx* c) {
+    DMesh<T> m;
+    m.nIC = c->nIC; m.nIF = c->nIF; m.nF = c->nF; m.nBF = c->nBF;
+    m.geo = (const T*)c->geo; m.vol = (const T*)c->vol;
+    m.sFace = c->sFace; m.sOther = c->sOther; m.sFlags = c->sFlags; m.sPair = c->sPair;
+    m.owner = c->owner; m.bPatch = c->bPatch; m.patches = c->patches; m.bcval = (const T*)c->bcval;
+    return m;
+}
+
+template <typename T> int upload(adfvm_ctx* ctx, void** dst, const std::vector<T>& v) {
+    HIPCHK(ctx, hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) HIPCHK(ctx, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return ADFVM_OK;
+}
+
+template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* d) {
+    const long nF = ctx->nF, nIC = ctx->nIC, nBF = ctx->nBF;
+    std::vector<T> geo((size_t)G_ROWS * nF);
+    for (long f = 0; f < nF; f++) {
+        geo[G_A * nF + f] = (T)d->areas[f];
+        for (int i = 0; i < 3; i++) geo[(G_NX + i) * nF + f] = (T)d->normals[3 * f + i];
+        geo[G_W * nF + f] = (T)d->weights[f];
+        geo[G_DELTA * nF + f] = (T)d->deltas[f];
+        for (int i = 0; i < 3; i++) geo[(G_DX + i) * nF + f] = (T)d->deltasUnit[3 * f + i];
+        geo[G_LW0 * nF + f] = (T)d->linearWeights[2 * f];
+        geo[G_LW1 * nF + f] = (T)d->linearWeights[2 * f + 1];
+        for (int i = 0; i < 3; i++) {
+            geo[(G_Q0X + i) * nF + f] = (T)d->quadraticWeights[6 * f + i];
+            geo[(G_Q1X + i) * nF + f] = (T)d->quadraticWeights[6 * f + 3 + i];
+        }
+    }
+    int rc;
+    if ((rc = upload(ctx, &ctx->geo, geo)) != ADFVM_OK) return rc;
+    std::vector<T> vol(nIC);
+    for (long c = 0; c < nIC; c++) vol[c] = (T)d->volumes[c];
+    if ((rc = upload(ctx, &ctx->vol, vol)) != ADFVM_OK) return rc;
+    // boundary-condition values
+    std::vector<T> vals;
+    for (int p = 0; p < d->nPatches; p++) {
+        const adfvm_patch_desc& P = d->patches[p];
+        DPatch& D = ctx->hpatches[p];
+        auto add = [&](const double* src, long cnt) -> long {
+            if (!src) return -1;
+            long off = (long)vals.size();
+            for (long i = 0; i < cnt; i++) vals.push_back((T)src[i]);
+            return off;
+        };
+        D.valU = add(P.valueU, 3L * P.nFaces);
+        D.valT = add(P.valueT, P.nFaces);
+        D.valp = add(P.valuep, P.nFaces);
+        D.valTt = add(P.Tt, P.nFaces);
+        D.valpt = add(P.pt, P.nFaces);
+        D.valDir = add(P.direction, 3L * P.nFaces);
+        D.hasDir = P.direction != nullptr;
+    }
+    if ((rc = upload(ctx, &ctx->bcval, vals)) != ADFVM_OK) return rc;
+    // work buffers
+    size_t st = 5 * nIC * sizeof(T), gr = 15 * nIC * sizeof(T);
+    HIPCHK(ctx, hipMalloc(&ctx->ghost, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
+    HIPCHK(ctx, hipMemset(ctx->ghost, 0, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->u1, st));
+    HIPCHK(ctx, hipMalloc(&ctx->u2, st));
+    HIPCHK(ctx, hipMalloc(&ctx->u3, st));
+    HIPCHK(ctx, hipMalloc(&ctx->grad0, gr));
+    HIPCHK(ctx, hipMalloc(&ctx->grad1, gr));
+    HIPCHK(ctx, hipMalloc(&ctx->grad2, gr));
+    HIPCHK(ctx, hipMalloc(&ctx->adj0, st));
+    HIPCHK(ctx, hipMalloc(&ctx->adj1, st));
+    HIPCHK(ctx, hipMalloc(&ctx->adjAcc, st));
+    HIPCHK(ctx, hipMalloc(&ctx->adjRbar, st));
+    HIPCHK(ctx, hipMalloc(&ctx->adjG, gr));
+    HIPCHK(ctx, hipMalloc(&ctx->adjW, st));
+    HIPCHK(ctx, hipMalloc(&ctx->adjGhost, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
+    return ADFVM_OK;
+}
+
+template <typename T> T* P(void* p) { return (T*)p; }
+
+// ghost refresh for state u (no-op for meshes with only cyclic patches)
+template <typename T> void launch_ghosts(adfvm_ctx* c, const T* u, hipStream_t s) {
+    if (!c->hasGhostKernel) return;
+    ghost_kernel<T><<<grid_for(c->nBF), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost));
+}
+
+template <typename T> void launch_grad(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
+    grad_kernel<T><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), grad);
+}
+
+template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* out, hipStream_t s) {
+    if (c->objective == ADFVM_OBJ_KINETIC_ENERGY) {
+        int g = grid_for(c->nIC);
+        ke_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), u, c->partial);
+        finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
+    } else if (c->objective == ADFVM_OBJ_DRAG) {
+        const DPatch& Pp = c->hpatches[c->objectivePatch];
+        int g = grid_for(Pp.n);
+        drag_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), Pp.start, Pp.n, c->partial);
+        finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
+    } else {
+        fill_kernel<<<1, 1, 0, s>>>(out, 1, 0.0);
+    }
+}
+
+struct Stage { double a0, ai, beta; };   // u_{i+1} = a0*u0 + ai*u_i - beta*(L-S)*dt
+void stages(int integrator, std::vector<Stage>& st) {
+    st.clear();
+    if (integrator == ADFVM_RK_EULER) { st.push_back({0, 1, 1}); return; }
+    st.push_back({0.0, 1.0, 1.0});
+    st.push_back({3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0});
+    st.push_back({1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0});
+}
+
+template <typename T>
+int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, double* obj_out, hipStream_t s) {
+    std::vector<Stage> st;
+    stages(c->integrator, st);
+    const int ns = (int)st.size();
+    T* bufs[3] = {P<T>(c->u1), P<T>(c->u2), u};
+    T* grads[3] = {P<T>(c->grad0), P<T>(c->grad1), P<T>(c->grad2)};
+    const T* ui = u;
+    const int objStage = ns > 1 ? 1 : 0;    // density.py:412-413
+    for (int i = 0; i < ns; i++) {
+        // the last stage writes u in place (each cell reads only its own u0
+        // entry); a single-stage scheme reads u's neighbours, so it goes
+        // through u1 and is copied back
+        T* out = (i == ns - 1 && ns > 1) ? u : bufs[i];
+        launch_ghosts<T>(c, ui, s);
+        if (i == objStage && obj_out) launch_objective<T>(c, ui, obj_out, s);
+        launch_grad<T>(c, ui, grads[i], s);
+        const bool wantDtc = (i == objStage) && dtc_out;
+        const T* u0 = (i > 0 && st[i].a0 != 0.0) ? (const T*)u : nullptr;
+        int g = grid_for(c->nIC);
+        flux_kernel<T, 0><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), ui, grads[i], P<T>(c->ghost), u0,
+                                               (T)st[i].a0, (T)st[i].ai, (T)(-st[i].beta), (T)dt, S, out, nullptr,
+                                               wantDtc ? c->partial : nullptr);
+        if (wantDtc) finalize_kernel<1><<<1, BLOCK, 0, s>>>(c->partial, g, dtc_out);
+        ui = out;
+    }
+    if (ns == 1) copy_kernel<T><<<grid_for(5L * c->nIC), BLOCK, 0, s>>>(5L * c->nIC, bufs[0], u);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- C ABI -----
+
+extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics* ph, adfvm_ctx** out) {
+    if (!d || !ph || !out) return ADFVM_ERR_INVALID;
+    *out = nullptr;
+    adfvm_ctx* c = new adfvm_ctx();
+    auto bad = [&](const std::string& m) { c->err = m; return ADFVM_ERR_INVALID; };
+    if (ph->precision != 4 && ph->precision != 8) { int r = bad("precision must be 4 or 8"); *out = c; return r; }
+    c->prec = ph->precision;
+    c->nIC = d->sizes.nInternalCells; c->nIF = d->sizes.nInternalFaces; c->nF = d->sizes.nFaces;
+    c->nBF = d->sizes.nBoundaryFaces; c->nC = d->sizes.nCells;
+    c->nPatches = d->nPatches;
+    c->integrator = ph->integrator;
+    c->objective = ph->objective;
+    c->objectivePatch = ph->objectivePatch;
+    c->phd.gamma = ph->gamma; c->phd.Cp = ph->Cp; c->phd.Cv = ph->Cp / ph->gamma; c->phd.Pr = ph->Pr;
+    c->phd.invCv = 1.0 / c->phd.Cv; c->phd.small = ph->small;
+    c->phd.mu0 = ph->mu0; c->phd.muA = ph->muA; c->phd.muTs = ph->muTs; c->phd.muModel = ph->muModel;
+    c->phd.riemann = ph->riemann; c->phd.boundaryRiemann = ph->boundaryRiemann;
+    c->phd.limit = ph->limiter;
+    *out = c;
+    if (c->nIC <= 0 || c->nF <= 0 || c->nBF < 0) return bad("bad mesh sizes");
+    if (c->objective == ADFVM_OBJ_DRAG && (c->objectivePatch < 0 || c->objectivePatch >= c->nPatches))
+        return bad("drag objective needs a valid patch index");
+    const long nIC = c->nIC, nF = c->nF, nIF = c->nIF, nBF = c->nBF;
+
+    // boundary face -> patch, patch table
+    std::vector<int> bPatch(nBF, -1);
+    c->hpatches.resize(d->nPatches);
+    for (int p = 0; p < d->nPatches; p++) {
+        const adfvm_patch_desc& Pd = d->patches[p];
+        if (Pd.startFace < nIF || Pd.startFace + Pd.nFaces > nF) return bad("patch outside boundary faces");
+        for (int k = 0; k < Pd.nFaces; k++) bPatch[Pd.startFace - nIF + k] = p;
+        DPatch& D = c->hpatches[p];
+        D.start = Pd.startFace; D.n = Pd.nFaces; D.fluxKind = Pd.fluxKind;
+        D.bcU = Pd.bcU; D.bcT = Pd.bcT; D.bcp = Pd.bcp; D.cbc = Pd.cbc;
+        const bool cyc = Pd.fluxKind == ADFVM_FLUX_COUPLED;
+        if (cyc && (Pd.neighbourPatch < 0 || Pd.neighbourPatch >= d->nPatches ||
+                    d->patches[Pd.neighbourPatch].nFaces != Pd.nFaces))
+            return bad("coupled patch without a matching neighbour patch");
+        if (cyc && (Pd.bcU != ADFVM_BC_CYCLIC || Pd.bcT != ADFVM_BC_CYCLIC || Pd.bcp != ADFVM_BC_CYCLIC))
+            return bad("coupled patches need cyclic conditions on U, T and p");
+        if (!cyc && (Pd.bcU == ADFVM_BC_CYCLIC || Pd.bcT == ADFVM_BC_CYCLIC || Pd.bcp == ADFVM_BC_CYCLIC))
+            return bad("cyclic condition on a non-coupled patch");
+        if (Pd.cbc == ADFVM_CBC_UPT && (!Pd.valueU || !Pd.valueT || !Pd.valuep))
+            return bad("CBC_UPT needs U0, T0, p0 values");
+        if (Pd.cbc == ADFVM_CBC_TOTAL_PT && (!Pd.Tt || !Pd.pt)) return bad("CBC_TOTAL_PT needs Tt and pt");
+        auto needs = [](int bc) { return bc == ADFVM_BC_FIXED_VALUE || bc == ADFVM_BC_CALCULATED; };
+        if (Pd.cbc == ADFVM_CBC_NONE && ((needs(Pd.bcU) && !Pd.valueU) || (needs(Pd.bcT) && !Pd.valueT) ||
+                                         (needs(Pd.bcp) && !Pd.valuep)))
+            return bad("fixedValue/calculated condition without values");
+        if (!cyc && Pd.nFaces > 0) c->hasGhostKernel = true;
+    }
+    for (long b = 0; b < nBF; b++)
+        if (bPatch[b] < 0) return bad("boundary face not covered by a patch");
+
+    // slot tables
+    std::vector<int> sFace(6 * nIC), sOther(6 * nIC), sPair(6 * nIC, -1);
+    std::vector<unsigned char> sFlags(6 * nIC);
+    std::vector<int> cellNb(6 * nIC);
+    for (long cc = 0; cc < nIC; cc++) {
+        for (int j = 0; j < 6; j++) {
+            int f = d->cellFaces[6 * cc + j];
+            if (f < 0 || f >= nF) return bad("cellFaces out of range");
+            int own = d->cellOwner[6 * cc + j];
+            int other, kind;
+            if (f < nIF) {
+                kind = SLOT_INTERNAL;
+                other = own ? d->neighbour[f] : d->owner[f];
+            } else {
+                if (!own) return bad("boundary face in a neighbour slot");
+                int b = f - nIF;
+                const adfvm_patch_desc& Pd = d->patches[bPatch[b]];
+                if (Pd.fluxKind == ADFVM_FLUX_COUPLED) {
+                    kind = SLOT_COUPLED;
+                    int nbStart = d->patches[Pd.neighbourPatch].startFace;
+                    int pf = nbStart + (f - Pd.startFace);
+                    other = d->owner[pf];
+                    sPair[j * nIC + cc] = pf;
+                } else {
+                    kind = Pd.fluxKind == ADFVM_FLUX_CHARACTERISTIC ? SLOT_CHARACTERISTIC : SLOT_BOUNDARY;
+                    other = -1 - b;
+                }
+            }
+            sFace[j * nIC + cc] = f;
+            sOther[j * nIC + cc] = other;
+            sFlags[j * nIC + cc] = (unsigned char)((own ? 1 : 0) | (kind << 1));
+            cellNb[j * nIC + cc] = d->cellNeighbours[6 * cc + j];
+        }
+    }
+    int rc;
+    std::vector<int> owner(d->owner, d->owner + nF), nbFull(d->neighbour, d->neighbour + nF);
+    if ((rc = upload(c, (void**)&c->sFace, sFace)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->sOther, sOther)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->sFlags, sFlags)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->sPair, sPair)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->cellNbFull, cellNb)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->owner, owner)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->nbFull, nbFull)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->bPatch, bPatch)) != ADFVM_OK) return rc;
+    rc = c->prec == 4 ? create_typed<float>(c, d) : create_typed<double>(c, d);
+    if (rc != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->patches, c->hpatches)) != ADFVM_OK) return rc;
+    c->nPartial = std::max(grid_for(std::max<long>(nIC, nF)), 1);
+    HIPCHK(c, hipMalloc((void**)&c->partial, c->nPartial * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->scal, 16 * sizeof(double)));
+    return ADFVM_OK;
+}
+
+extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
+    if (!c) return;
+    void* ptrs[] = {c->geo, c->vol, c->bcval, c->sFace, c->sOther, c->sFlags, c->sPair, c->owner, c->bPatch,
+                    c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->u3, c->grad0, c->grad1,
+                    c->grad2, c->adj0, c->adj1, c->adjAcc, c->adjRbar, c->adjG, c->adjW, c->adjGhost,
+                    c->partial, c->scal};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+extern "C" const char* adfvm_last_error(const adfvm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+extern "C" int adfvm_ctx_precision(const adfvm_ctx* c) { return c ? c->prec : 0; }
+
+#define DISPATCH(c, CALL_F, CALL_D) ((c)->prec == 4 ? (CALL_F) : (CALL_D))
+
+extern "C" int adfvm_primal_step(adfvm_ctx* c, void* u, double dt, const void* S, double* dtc_out,
+                                 double* obj_out, void* stream) {
+    if (!c || !u) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    return DISPATCH(c, primal_step_t<float>(c, (float*)u, dt, (const float*)S, dtc_out, obj_out, s),
+                    primal_step_t<double>(c, (double*)u, dt, (const double*)S, dtc_out, obj_out, s));
+}
+
+namespace {
+template <typename T> int gradients_t(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
+    launch_ghosts<T>(c, u, s);
+    launch_grad<T>(c, u, grad, s);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+template <typename T> int residual_t(adfvm_ctx* c, const T* u, T* R, T* dtcCell, hipStream_t s) {
+    launch_ghosts<T>(c, u, s);
+    launch_grad<T>(c, u, P<T>(c->grad0), s);
+    flux_kernel<T, 1><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->grad0),
+                                                          P<T>(c->ghost), nullptr, T(0), T(0), T(0), T(0),
+                                                          nullptr, R, dtcCell, nullptr);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+template <typename T> int objective_t(adfvm_ctx* c, const T* u, double* out, hipStream_t s) {
+    launch_ghosts<T>(c, u, s);
+    launch_objective<T>(c, u, out, s);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+template <typename T> int prim_t(adfvm_ctx* c, const T* u, T* prim, hipStream_t s) {
+    launch_ghosts<T>(c, u, s);
+    prim_ghost_kernel<T><<<grid_for((long)c->nIC + c->nBF), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u,
+                                                                           P<T>(c->ghost), prim);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+}  // namespace
+
+extern "C" int adfvm_gradients(adfvm_ctx* c, const void* u, void* grad, void* stream) {
+    if (!c || !u || !grad) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    return DISPATCH(c, gradients_t<float>(c, (const float*)u, (float*)grad, s),
+                    gradients_t<double>(c, (const double*)u, (double*)grad, s));
+}
+
+extern "C" int adfvm_residual(adfvm_ctx* c, const void* u, void* R, void* dtcCell, void* stream) {
+    if (!c || !u || !R) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    return DISPATCH(c, residual_t<float>(c, (const float*)u, (float*)R, (float*)dtcCell, s),
+                    residual_t<double>(c, (const double*)u, (double*)R, (double*)dtcCell, s));
+}
+
+extern "C" int adfvm_objective(adfvm_ctx* c, const void* u, double* out, void* stream) {
+    if (!c || !u || !out) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    return DISPATCH(c, objective_t<float>(c, (const float*)u, out, s), objective_t<double>(c, (const double*)u, out, s));
+}
+
+extern "C" int adfvm_primitive_ghosts(adfvm_ctx* c, const void* u, void* prim, void* stream) {
+    if (!c || !u || !prim) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    return DISPATCH(c, prim_t<float>(c, (const float*)u, (float*)prim, s), prim_t<double>(c, (const double*)u, (double*)prim, s));
+}
+
+extern "C" int adfvm_grad_cell(adfvm_ctx* c, const void* phi, int32_t k, void* grad, void* stream) {
+    if (!c || !phi || !grad || k <= 0) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    long nC = (long)c->nIC + c->nBF;
+    if (c->prec == 4)
+        grad_field_kernel<float><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<float>(c), c->cellNbFull, (const float*)phi, k, nC, (float*)grad);
+    else
+        grad_field_kernel<double><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<double>(c), c->cellNbFull, (const double*)phi, k, nC, (double*)grad);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_interp_second_order(adfvm_ctx* c, const void* phi, const void* grad, int32_t k, int32_t swap,
+                                         void* out, void* stream) {
+    if (!c || !phi || !grad || !out || k <= 0 || (swap != 0 && swap != 1)) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    long nC = (long)c->nIC + c->nBF;
+    if (c->prec == 4)
+        interp_kernel<float><<<grid_for(c->nF), BLOCK, 0, s>>>(dmesh<float>(c), c->nbFull, (const float*)phi, (const float*)grad, k, nC, swap, (float*)out);
+    else
+        interp_kernel<double><<<grid_for(c->nF), BLOCK, 0, s>>>(dmesh<double>(c), c->nbFull, (const double*)phi, (const double*)grad, k, nC, swap, (double*)out);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_dot(adfvm_ctx* c, const void* a, const void* b, int64_t n, double* out, void* stream) {
+    if (!c || !a || !b || !out || n < 0) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    int g = std::min<long>(std::max<long>(grid_for(n), 1), c->nPartial);
+    if (c->prec == 4) dot_kernel<float><<<g, BLOCK, 0, s>>>((const float*)a, (const float*)b, n, c->partial);
+    else dot_kernel<double><<<g, BLOCK, 0, s>>>((const double*)a, (const double*)b, n, c->partial);
+    finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_adjoint_energy(adfvm_ctx* c, const void* ua, const void* u, double* out, void* stream) {
+    if (!c || !ua || !u || !out) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    int g = grid_for(c->nIC);
+    if (c->prec == 4) energy_kernel<float><<<g, BLOCK, 0, s>>>(dmesh<float>(c), phys_cast<float>(c->phd), (const float*)ua, (const float*)u, c->partial);
+    else energy_kernel<double><<<g, BLOCK, 0, s>>>(dmesh<double>(c), phys_cast<double>(c->phd), (const double*)ua, (const double*)u, c->partial);
+    finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_pack_state(adfvm_ctx* c, const void* rho, const void* rhoU, const void* rhoE, void* u, void* stream) {
+    if (!c || !rho || !rhoU || !rhoE || !u) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->prec == 4) pack_kernel<float><<<grid_for(c->nIC), BLOCK, 0, s>>>(c->nIC, (const float*)rho, (const float*)rhoU, (const float*)rhoE, (float*)u);
+    else pack_kernel<double><<<grid_for(c->nIC), BLOCK, 0, s>>>(c->nIC, (const double*)rho, (const double*)rhoU, (const double*)rhoE, (double*)u);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_unpack_state(adfvm_ctx* c, const void* u, void* rho, void* rhoU, void* rhoE, void* stream) {
+    if (!c || !rho || !rhoU || !rhoE || !u) return ADFVM_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->prec == 4) unpack_kernel<float><<<grid_for(c->nIC), BLOCK, 0, s>>>(c->nIC, (const float*)u, (float*)rho, (float*)rhoU, (float*)rhoE);
+    else unpack_kernel<double><<<grid_for(c->nIC), BLOCK, 0, s>>>(c->nIC, (const double*)u, (double*)rho, (double*)rhoU, (double*)rhoE);
+    HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}

@@ -1,0 +1,173 @@
+"""Mesh: adFVM's mesh data model (adFVM/mesh.py), built by libadfvm's C++
+index/geometry builder.
+
+Attribute names, shapes and the tensor/scalar argument lists follow the
+reference so that code written against `adFVM.mesh.Mesh` reads the same:
+`gradFields`, `intFields`, `constants` (mesh.py:27-37), `getTensor` /
+`getScalar` (:870-881), `sortedPatches` (:100-101), patch ranges
+(:713-736), `defaultBoundary` / `calculatedBoundary` (:702-721).
+"""
+import copy
+from collections import OrderedDict
+
+import numpy as np
+
+from . import config
+from . import _lib
+from .meshgen import MeshData
+
+
+class Mesh(object):
+    intFields = ['owner', 'neighbour', 'cellFaces', 'cellNeighbours', 'cellOwner']
+    gradFields = ['areas', 'volumesL', 'volumesR', 'weights', 'deltas', 'normals', 'deltasUnit',
+                  'linearWeights', 'quadraticWeights', 'volumes']
+    BCFields = ['startFace', 'nFaces', 'cellStartFace']
+    constants = ['nCells', 'nFaces', 'nInternalCells', 'nInternalFaces',
+                 'nLocalCells', 'nRemoteCells', 'nLocalFaces', 'nGhostCells']
+
+    def __init__(self):
+        self.boundary = OrderedDict()
+        self.caseDir = None
+        self.case = ''
+
+    # ------------------------------------------------------------ creation --
+    @classmethod
+    def create(cls, caseDir=None, currTime='constant'):
+        """Read an OpenFOAM polyMesh and build (adFVM/mesh.py:70-81)."""
+        from . import foam
+        self = cls()
+        self.caseDir = caseDir
+        self.case = caseDir if caseDir.endswith('/') else caseDir + '/'
+        self.build(foam.read_polymesh(caseDir, currTime))
+        return self
+
+    @classmethod
+    def from_data(cls, meshdata, caseDir=None):
+        """Build from a generated MeshData (adfvm_amd.meshgen)."""
+        self = cls()
+        if caseDir is not None:
+            self.caseDir = caseDir
+            self.case = caseDir if caseDir.endswith('/') else caseDir + '/'
+        self.build(meshdata)
+        return self
+
+    def build(self, md):
+        """Mesh.build (adFVM/mesh.py:83-127) with cmesh.build replaced by
+        adfvm_mesh_build."""
+        lib = _lib.load()
+        points = np.ascontiguousarray(md.points, np.float64)
+        faces4 = np.ascontiguousarray(md.faces, np.int32)
+        owner = np.ascontiguousarray(md.owner, np.int32)
+        neigh = np.ascontiguousarray(md.neighbour, np.int32)
+        names = list(md.boundary.keys())
+        bnd = OrderedDict((k, dict(v)) for k, v in md.boundary.items())
+        ptype = np.array([_lib.PATCH_CYCLIC if bnd[n]['type'] in config.cyclicPatches else
+                          _lib.PATCH_PROCESSOR if bnd[n]['type'] in config.processorPatches else
+                          _lib.PATCH_OTHER for n in names], np.int32)
+        pstart = np.array([bnd[n]['startFace'] for n in names], np.int32)
+        pn = np.array([bnd[n]['nFaces'] for n in names], np.int32)
+        pnb = np.array([names.index(bnd[n]['neighbourPatch']) if ptype[i] == _lib.PATCH_CYCLIC else -1
+                        for i, n in enumerate(names)], np.int32)
+        inp = _lib.MeshInput(len(points), len(owner), len(neigh), len(names),
+                             _lib.ptr(points, _lib.C.c_double), _lib.ptr(faces4, _lib.C.c_int32),
+                             _lib.ptr(owner, _lib.C.c_int32), _lib.ptr(neigh, _lib.C.c_int32),
+                             _lib.ptr(pstart, _lib.C.c_int32), _lib.ptr(pn, _lib.C.c_int32),
+                             _lib.ptr(ptype, _lib.C.c_int32), _lib.ptr(pnb, _lib.C.c_int32))
+        sz = _lib.MeshSizes()
+        _lib.check(lib.adfvm_mesh_sizes(_lib.C.byref(inp), _lib.C.byref(sz)))
+        nF, nIF, nIC, nC = sz.nFaces, sz.nInternalFaces, sz.nInternalCells, sz.nCells
+        i32, f64 = np.int32, np.float64
+        arr = dict(cellFaces=np.zeros((nIC, 6), i32), cells=np.zeros((nIC, 8), i32),
+                   neighbour=np.zeros(nF, i32), cellOwner=np.zeros((nIC, 6), i32),
+                   cellNeighbours=np.zeros((nIC, 6), i32), cellNeighboursMatOp=np.zeros((nIC, 6), i32),
+                   normals=np.zeros((nF, 3), f64), faceCentres=np.zeros((nF, 3), f64),
+                   areas=np.zeros((nF, 1), f64), cellCentres=np.zeros((nC, 3), f64),
+                   volumes=np.zeros((nIC, 1), f64), deltas=np.zeros((nF, 1), f64),
+                   deltasUnit=np.zeros((nF, 3), f64), weights=np.zeros((nF, 1), f64),
+                   linearWeights=np.zeros((nF, 2), f64), quadraticWeights=np.zeros((nF, 2, 3), f64),
+                   volumesL=np.zeros((nF, 1), f64), volumesR=np.zeros((nIF, 1), f64),
+                   nLocalCells=np.zeros(1, i32))
+        out = _lib.MeshOutput(**{k: _lib.ptr(v, _lib.C.c_int32 if v.dtype == i32 else _lib.C.c_double)
+                                 for k, v in arr.items()})
+        _lib.check(lib.adfvm_mesh_build(_lib.C.byref(inp), _lib.C.byref(out)))
+
+        self.points = points
+        self.faces = np.hstack([np.full((nF, 1), 4, i32), faces4])   # reference layout: nPoints first
+        self.owner = owner
+        self.nInternalFaces, self.nFaces = nIF, nF
+        self.nBoundaryFaces = sz.nBoundaryFaces
+        self.nInternalCells, self.nGhostCells, self.nCells = nIC, sz.nGhostCells, nC
+        self.nLocalCells = int(arr.pop('nLocalCells')[0])
+        for k, v in arr.items():
+            setattr(self, k, v)
+        self.boundary = bnd
+        self.splitPatches()
+        self.nRemoteCells = self.nCells - self.nLocalCells
+        self.nLocalFaces = self.nLocalCells - self.nInternalCells + self.nInternalFaces
+        self.defaultBoundary = self.getDefaultBoundary()
+        self.calculatedBoundary = self.getCalculatedBoundary()
+        for name in self.boundary:
+            patch = self.boundary[name]
+            if patch['type'] in config.cyclicPatches:
+                s = self.boundary[patch['neighbourPatch']]['startFace']
+                patch['loc_neighbourIndices'] = self.owner[s:s + patch['nFaces']]
+        return self
+
+    def splitPatches(self):
+        """cellStartFace and local/remote split, adFVM/mesh.py:238-249."""
+        ids = sorted(self.boundary.keys(), key=lambda x: (self.boundary[x]['startFace'], self.boundary[x]['nFaces']))
+        local, remote = [], []
+        for pid in ids:
+            b = self.boundary[pid]
+            b['cellStartFace'] = b['startFace'] - self.nInternalFaces + self.nInternalCells
+            (remote if b['type'] in config.processorPatches else local).append(pid)
+        self.localPatches, self.remotePatches = local, remote
+        self.patches = local + remote
+        self.sortedPatches = sorted(local)
+        return local, remote
+
+    # ------------------------------------------------------------- queries --
+    def getDefaultBoundary(self):
+        """adFVM/mesh.py:702-711."""
+        b = OrderedDict()
+        for pid in self.boundary:
+            t = self.boundary[pid]['type']
+            b[pid] = {'type': t if t in config.defaultPatches else 'zeroGradient'}
+        return b
+
+    def getCalculatedBoundary(self):
+        """adFVM/mesh.py:713-721."""
+        b = OrderedDict()
+        for pid in self.boundary:
+            t = self.boundary[pid]['type']
+            b[pid] = {'type': t if t in config.coupledPatches else 'calculated'}
+        return b
+
+    def getPatchFaceRange(self, patchID):
+        p = self.boundary[patchID]
+        return p['startFace'], p['startFace'] + p['nFaces'], p['nFaces']
+
+    def getPatchCellRange(self, patchID, faceRange=None):
+        s, e, n = faceRange if faceRange else self.getPatchFaceRange(patchID)
+        d = self.nInternalCells - self.nInternalFaces
+        return s + d, e + d, n
+
+    def getPatchFaceCellRange(self, patchID):
+        s, e, n = self.getPatchFaceRange(patchID)
+        cs, ce, _ = self.getPatchCellRange(patchID, (s, e, n))
+        return s, e, cs, ce, n
+
+    def getTensor(self):
+        """Float arrays then int arrays, adFVM/mesh.py:870-872."""
+        return [getattr(self, a) for a in Mesh.gradFields] + [getattr(self, a) for a in Mesh.intFields]
+
+    def getScalar(self):
+        """Constants then (startFace, nFaces, cellStartFace) per sorted patch,
+        adFVM/mesh.py:874-881."""
+        out = [getattr(self, a) for a in Mesh.constants]
+        for pid in self.sortedPatches:
+            out.extend([self.boundary[pid][a] for a in Mesh.BCFields])
+        return out
+
+    def copy(self):
+        return copy.deepcopy(self)

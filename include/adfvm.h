@@ -1,0 +1,256 @@
+/* adfvm.h -- C ABI of libadfvm.so, the MI355X implementation of adFVM's
+ * compressible finite-volume step and its discrete adjoint.
+ *
+ * The reference's operator boundary is the adpy `Function` objects it builds
+ * and calls as `outputs = self.map(*inputs)`:
+ *   primal step   adFVM/density.py:80-105 (built), adFVM/solver.py:312-323 (called)
+ *   adjoint step  apps/adjoint.py:94-126  (built), apps/adjoint.py:268-291 (called)
+ *   init/ghosts   adFVM/density.py:64-78, :140-143
+ * plus the native helpers those Functions and the mesh rely on:
+ *   cmesh.build / buildBeforeWrite   adFVM/cpp/cmesh.cpp:3-270, :427-432
+ *   cmesh.computeSensitivity         adFVM/cpp/cmesh.cpp:309-340
+ *   cmesh.computeEnergy              adFVM/cpp/cmesh.cpp:342-425
+ *   external_init / external_exit    adFVM/cpp/external.cpp:21-51
+ *
+ * Conventions: plain pointers and sizes.  "device" pointers are HIP device
+ * memory (for example a torch tensor's data_ptr()); "host" pointers are CPU
+ * memory.  Floating-point state arrays use the context's precision (float
+ * when precision == 4, double when 8) and structure-of-arrays layout:
+ *   state  u[5][nInternalCells]   rows rho, rhoUx, rhoUy, rhoUz, rhoE
+ * Streams are hipStream_t passed as void* (NULL = default stream).  Every
+ * function returns ADFVM_OK or an error code; adfvm_last_error() /
+ * adfvm_mesh_last_error() give the message.  Calls on one context are not
+ * reentrant.
+ */
+#ifndef ADFVM_H
+#define ADFVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADFVM_OK 0
+#define ADFVM_ERR_INVALID 1
+#define ADFVM_ERR_HIP 2
+#define ADFVM_ERR_UNSUPPORTED 3
+
+/* patch topology (adFVM/config.py:388-391) */
+#define ADFVM_PATCH_OTHER 0
+#define ADFVM_PATCH_CYCLIC 1
+#define ADFVM_PATCH_PROCESSOR 2
+
+/* ------------------------------------------------------------------ mesh --- */
+
+typedef struct {
+    int32_t nPoints, nFaces, nInternalFaces, nPatches;
+    const double* points;          /* host, nPoints x 3 */
+    const int32_t* faces;          /* host, nFaces x 4 quad point indices */
+    const int32_t* owner;          /* host, nFaces */
+    const int32_t* neighbour;      /* host, nInternalFaces */
+    const int32_t* patchStartFace; /* host, nPatches (file order) */
+    const int32_t* patchNFaces;
+    const int32_t* patchType;      /* ADFVM_PATCH_* */
+    const int32_t* patchNeighbour; /* cyclic partner patch index, else -1 */
+} adfvm_mesh_input;
+
+typedef struct {
+    int32_t nInternalFaces, nFaces, nBoundaryFaces, nInternalCells, nGhostCells, nCells;
+} adfvm_mesh_sizes_t;
+
+/* caller-allocated host outputs; `cells` and `cellNeighboursMatOp` may be NULL */
+typedef struct {
+    int32_t* cellFaces;          /* nInternalCells x 6 */
+    int32_t* cells;              /* nInternalCells x 8 */
+    int32_t* neighbour;          /* nFaces, ghost-extended */
+    int32_t* cellOwner;          /* nInternalCells x 6 */
+    int32_t* cellNeighbours;     /* nInternalCells x 6, ghost indices for boundary faces */
+    int32_t* cellNeighboursMatOp;/* nInternalCells x 6, -1 for boundary faces */
+    double* normals;             /* nFaces x 3 */
+    double* faceCentres;         /* nFaces x 3 */
+    double* areas;               /* nFaces */
+    double* cellCentres;         /* nCells x 3 (ghost centres included) */
+    double* volumes;             /* nInternalCells */
+    double* deltas;              /* nFaces */
+    double* deltasUnit;          /* nFaces x 3 */
+    double* weights;             /* nFaces */
+    double* linearWeights;       /* nFaces x 2 */
+    double* quadraticWeights;    /* nFaces x 2 x 3 */
+    double* volumesL;            /* nFaces */
+    double* volumesR;            /* nInternalFaces */
+    int32_t* nLocalCells;        /* 1 */
+} adfvm_mesh_output;
+
+/* Sizes, adFVM/mesh.py:517-523 (Mesh.populateSizes). Host only. */
+int adfvm_mesh_sizes(const adfvm_mesh_input* in, adfvm_mesh_sizes_t* out);
+
+/* Index construction and FP64 geometry in one call; replaces
+ * cmesh.buildBeforeWrite (adFVM/cpp/cmesh.cpp:3-63), cmesh.build
+ * (cmesh.cpp:65-270) with its createGhostCells callback (adFVM/mesh.py:758-819)
+ * and volumesL/R (mesh.py:108-109).  Host only; no GPU needed. */
+int adfvm_mesh_build(const adfvm_mesh_input* in, adfvm_mesh_output* out);
+const char* adfvm_mesh_last_error(void);
+
+/* ---------------------------------------------------------------- solver --- */
+
+/* per-field ghost rules, adFVM/BCs.py */
+#define ADFVM_BC_CYCLIC 0        /* BCs.py:86-95 */
+#define ADFVM_BC_ZERO_GRADIENT 1 /* BCs.py:97-104; also empty, inletOutlet (:135-137) */
+#define ADFVM_BC_SYMMETRY 2      /* BCs.py:120-133; symmetryPlane, slip */
+#define ADFVM_BC_FIXED_VALUE 3   /* BCs.py:111-118 */
+#define ADFVM_BC_CALCULATED 4    /* BCs.py:74-84: ghost keeps its value */
+
+/* patch flux treatment, adFVM/density.py:391-400 */
+#define ADFVM_FLUX_COUPLED 0        /* _coupledFlux, cyclic */
+#define ADFVM_FLUX_CHARACTERISTIC 1 /* _characteristicFlux */
+#define ADFVM_FLUX_BOUNDARY 2       /* _boundaryFlux */
+
+/* characteristic ghost rules, adFVM/BCs.py:140-184 */
+#define ADFVM_CBC_NONE 0
+#define ADFVM_CBC_UPT 1
+#define ADFVM_CBC_TOTAL_PT 2
+
+typedef struct {
+    int32_t startFace, nFaces;
+    int32_t fluxKind;        /* ADFVM_FLUX_* */
+    int32_t neighbourPatch;  /* cyclic partner index into the patch array, else -1 */
+    int32_t bcU, bcT, bcp;   /* ADFVM_BC_* */
+    int32_t cbc;             /* ADFVM_CBC_* (overrides bcU/bcT/bcp) */
+    /* host FP64 value arrays, nullable: fixedValue/calculated/CBC_UPT values
+     * (U nFaces x 3, T and p nFaces); CBC_TOTAL_PT Tt, pt (nFaces) and
+     * direction (nFaces x 3, NULL = face normal, BCs.py:172-175) */
+    const double* valueU;
+    const double* valueT;
+    const double* valuep;
+    const double* Tt;
+    const double* pt;
+    const double* direction;
+} adfvm_patch_desc;
+
+#define ADFVM_RIEMANN_ROE 0  /* adFVM/riemann.py:21-70 */
+#define ADFVM_RIEMANN_LF 1   /* adFVM/riemann.py:6-18 */
+#define ADFVM_MU_CONSTANT 0
+#define ADFVM_MU_SUTHERLAND 1 /* muA*T^1.5/(T+muTs), adFVM/density.py:27 */
+#define ADFVM_RK_SSPRK3 0     /* adFVM/timestep.py:17-25 */
+#define ADFVM_RK_EULER 1      /* adFVM/timestep.py:9-13 */
+#define ADFVM_OBJ_NONE 0
+#define ADFVM_OBJ_KINETIC_ENERGY 1 /* sum_c 0.5 |rhoU|^2/rho V_c */
+#define ADFVM_OBJ_DRAG 2           /* templates/cylinder_test.py:9-37 on one patch */
+
+typedef struct {
+    int32_t precision;             /* 4 (float kernels) or 8 (double) */
+    double gamma, Cp, Pr;          /* adFVM/density.py:25-28 */
+    int32_t muModel;
+    double mu0, muA, muTs;
+    int32_t riemann, boundaryRiemann;
+    double small;                  /* config.SMALL, adFVM/config.py:170-178 */
+    int32_t integrator;            /* ADFVM_RK_* */
+    int32_t objective;             /* ADFVM_OBJ_* */
+    int32_t objectivePatch;        /* patch index for ADFVM_OBJ_DRAG */
+    int32_t limiter;               /* 0: interp.secondOrder as in the reference (unlimited);
+                                      1: reconstructed values clamped between the two cell
+                                      values (an addition; needed for Sod-type jumps) */
+} adfvm_physics;
+
+typedef struct {
+    adfvm_mesh_sizes_t sizes;
+    /* host arrays as produced by adfvm_mesh_build */
+    const int32_t* owner;          /* nFaces */
+    const int32_t* neighbour;      /* nFaces (ghost-extended) */
+    const int32_t* cellFaces;      /* nInternalCells x 6 */
+    const int32_t* cellOwner;      /* nInternalCells x 6 */
+    const int32_t* cellNeighbours; /* nInternalCells x 6 */
+    const double* areas;
+    const double* normals;
+    const double* weights;
+    const double* deltas;
+    const double* deltasUnit;
+    const double* linearWeights;
+    const double* quadraticWeights;
+    const double* volumes;
+    int32_t nPatches;
+    const adfvm_patch_desc* patches;
+} adfvm_solver_desc;
+
+typedef struct adfvm_ctx adfvm_ctx;
+
+/* Uploads the mesh to the current HIP device in structure-of-arrays form,
+ * builds the cell->face slot tables with cyclic neighbours resolved to
+ * internal cells, and allocates the step's work buffers.  Replaces
+ * Function.initialize -> external_init (adFVM/cpp/external.cpp:21-42). */
+int adfvm_ctx_create(const adfvm_solver_desc* desc, const adfvm_physics* phys, adfvm_ctx** out);
+void adfvm_ctx_destroy(adfvm_ctx* ctx);
+const char* adfvm_last_error(const adfvm_ctx* ctx);
+int adfvm_ctx_precision(const adfvm_ctx* ctx);
+
+/* One explicit time step in place: u <- step(u).  The reference's primal
+ * Function (density.py:97-105: SSP-RK3 via timestep.py:27-55 of
+ * RCF.equation, density.py:339-414).  source: device [5][nIC] or NULL.
+ * dtc_out (device double, nullable): max_c sum_f (|Un|+a)A/V evaluated on the
+ * first intermediate stage, density.py:405-413.  obj_out (device double,
+ * nullable): the objective on the same stage, in FP64. */
+int adfvm_primal_step(adfvm_ctx* ctx, void* u, double dt, const void* source,
+                      double* dtc_out, double* obj_out, void* stream);
+
+/* Reverse of one time step (the primal_grad Function, apps/adjoint.py:94-126,
+ * called at :268-291 with dtca = 0).  u_n: device [5][nIC], the primal state
+ * at the start of the step (unchanged).  ua: device [5][nIC]; in: adjoint of
+ * the step's output, out: adjoint of its input, d(obja*J_n + <ua_in,u_{n+1}>)/du_n.
+ * source_grad (device [5][nIC], nullable): the source-term gradient of the
+ * same functional is ADDED to it (parameter 'source', apps/adjoint.py:104-106). */
+int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* source,
+                       void* ua, double obja, void* source_grad, void* stream);
+
+/* Operator-level entry points (the hot path's pieces, used by tests). */
+
+/* Gauss cell gradients of the primitive fields of u, adFVM/op.py:45-63 via
+ * density.py:245-251.  grad: device [15][nIC], rows dU_i/dx_k (i-major,
+ * 9 rows), dT/dx_k (3), dp/dx_k (3). */
+int adfvm_gradients(adfvm_ctx* ctx, const void* u, void* grad, void* stream);
+
+/* Right-hand side R(u) = (1/V) sum_f F.A (div of the face fluxes,
+ * adFVM/op.py:12-29 from density.py:253-331,388-403).  R: device [5][nIC];
+ * dtc_cell: device [nIC] or NULL. */
+int adfvm_residual(adfvm_ctx* ctx, const void* u, void* R, void* dtc_cell, void* stream);
+
+/* ubar = (dR/du)^T rbar.  rbar, ubar: device [5][nIC]. */
+int adfvm_residual_vjp(adfvm_ctx* ctx, const void* u, const void* rbar, void* ubar, void* stream);
+
+/* Objective on a state (after ghost update), FP64 result in obj_out (device). */
+int adfvm_objective(adfvm_ctx* ctx, const void* u, double* obj_out, void* stream);
+
+/* Primitive fields with ghost cells, the 'init' Function's boundary update
+ * (density.py:64-78): prim device [5][nCells] rows Ux,Uy,Uz,T,p. */
+int adfvm_primitive_ghosts(adfvm_ctx* ctx, const void* u, void* prim, void* stream);
+
+/* Gauss gradient of an arbitrary cell field given on all nCells (ghost values
+ * included, no boundary update), the reference unit-test operator
+ * (tests/test_op.py:20-91 -> op.gradCell).  phi: device [k][nCells];
+ * grad: device [k*3][nIC]. */
+int adfvm_grad_cell(adfvm_ctx* ctx, const void* phi, int32_t k, void* grad, void* stream);
+
+/* Second-order face reconstruction interp.secondOrder (adFVM/interp.py:20-28),
+ * swap 0 (owner side) or 1: phi device [k][nCells], grad device [k*3][nCells],
+ * out device [k][nFaces]. */
+int adfvm_interp_second_order(adfvm_ctx* ctx, const void* phi, const void* grad, int32_t k,
+                              int32_t swap, void* out, void* stream);
+
+/* Reductions with FP64 accumulation in a fixed order.
+ * sensitivity = sum_j a[j]*b[j] (cmesh.cpp:309-340); n scalars, device. */
+int adfvm_dot(adfvm_ctx* ctx, const void* a, const void* b, int64_t n, double* out, void* stream);
+/* adjoint energy sum_c w^T A0 w V_c with w = ua/V (cmesh.cpp:342-425). */
+int adfvm_adjoint_energy(adfvm_ctx* ctx, const void* ua, const void* u, double* out, void* stream);
+
+/* Layout helpers: reference layout (rho nIC, rhoU nIC x 3 row-major,
+ * rhoE nIC; device, context precision) <-> state u[5][nIC]. */
+int adfvm_pack_state(adfvm_ctx* ctx, const void* rho, const void* rhoU, const void* rhoE,
+                     void* u, void* stream);
+int adfvm_unpack_state(adfvm_ctx* ctx, const void* u, void* rho, void* rhoU, void* rhoE,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADFVM_H */

@@ -1,0 +1,480 @@
+"""ORACLE (test infrastructure only): CPU float64 restatement of adFVM's
+compressible finite-volume step, written with torch tensor ops so that
+torch.autograd supplies the exact discrete adjoint (the role adpy's reverse
+mode plays for the reference, apps/adjoint.py:101).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module, and only as the checker; the product runs HIP kernels.
+
+Each function cites the reference lines it restates.  Face->cell
+accumulation runs over each cell's cellFaces slots in order (owned faces in
+face order, then neighbour faces), a fixed order; the reference leaves the
+order of its scatter-add (`Tensor.collate`) to adpy, so bitwise agreement with
+the reference's sums is not claimed.
+
+Inputs: an oracle mesh (oracle.mesh_oracle.build) and a case spec:
+  bcs[patch][field] = {'type': 'cyclic'|'zeroGradient'|'empty'|'inletOutlet'|
+                       'symmetryPlane'|'slip'|'fixedValue'|'calculated'|
+                       'CBC_UPT'|'CBC_TOTAL_PT', 'value': array, ...}
+  (missing fields default to the mesh's default boundary, mesh.py:702-711)
+"""
+import numpy as np
+import torch
+
+DT = torch.float64
+CYCLIC_TYPES = ('cyclic', 'slidingPeriodic1D')
+COPY_TYPES = ('zeroGradient', 'empty', 'inletOutlet')          # BCs.py:97-104,135-137
+SYMMETRY_TYPES = ('symmetryPlane', 'slip')                       # BCs.py:120-135
+DEFAULT_TYPES = ('cyclic', 'slidingPeriodic1D', 'processor', 'processorCyclic', 'symmetryPlane', 'empty')
+
+
+def T_(a, dtype=DT):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype)
+
+
+def bvalue(v, n, k):
+    """A patch value given uniform (k numbers) or per face (n x k)."""
+    a = np.asarray(v, np.float64)
+    if a.size == k:
+        a = np.tile(a.reshape(1, k), (n, 1))
+    return T_(a.reshape(n, k))
+
+
+class Case(object):
+    """Physics constants and boundary spec; defaults from RCF.defaultConfig
+    (adFVM/density.py:22-39)."""
+
+    def __init__(self, mesh, bcs=None, gamma=1.4, Cp=1004.5, Pr=0.7, mu=None,
+                 riemann='roe', boundaryRiemann='roe', small=1e-30, objective=None,
+                 integrator='SSPRK', limiter=False):
+        self.mesh = mesh
+        self.gamma, self.Cp, self.Pr = gamma, Cp, Pr
+        self.Cv = Cp / gamma                      # density.py:44
+        self.R = Cp - self.Cv                     # density.py:45
+        self.mu = mu if mu is not None else (lambda T: 1.4792e-06 * T ** 1.5 / (T + 116.))
+        self.riemann, self.boundaryRiemann = riemann, boundaryRiemann
+        self.small = small
+        self.objective = objective
+        self.integrator = integrator
+        self.limiter = limiter
+        m = mesh
+        self.nIC, self.nIF, self.nF, self.nC = m.nInternalCells, m.nInternalFaces, m.nFaces, m.nCells
+        self.t = {k: T_(getattr(m, k)) for k in ['areas', 'normals', 'weights', 'deltas', 'deltasUnit',
+                                                    'linearWeights', 'quadraticWeights', 'volumes']}
+        self.t['volumes'] = self.t['volumes'].reshape(-1)
+        self.t['areas'] = self.t['areas'].reshape(-1)
+        self.t['weights'] = self.t['weights'].reshape(-1)
+        self.t['deltas'] = self.t['deltas'].reshape(-1)
+        self.t['quadraticWeights'] = self.t['quadraticWeights'].reshape(-1, 2, 3)
+        self.owner = torch.as_tensor(m.owner.astype(np.int64))
+        self.neighbour = torch.as_tensor(m.neighbour.astype(np.int64))
+        self.cellFaces = torch.as_tensor(m.cellFaces.astype(np.int64))
+        self.cellOwner = torch.as_tensor(m.cellOwner.astype(np.int64))
+        self.cellNeighbours = torch.as_tensor(m.cellNeighbours.astype(np.int64))
+        # patches: sorted (alphabetical) local patches, mesh.py:100-101
+        self.patches = []
+        bcs = bcs or {}
+        for name in sorted(m.boundary.keys()):
+            b = m.boundary[name]
+            spec = dict(bcs.get(name, {}))
+            for f in ('U', 'T', 'p'):
+                if f not in spec:
+                    spec[f] = {'type': b['type'] if b['type'] in DEFAULT_TYPES else 'zeroGradient'}
+            ptype = b['type']
+            cbc = spec['p']['type'] if spec['p']['type'] in ('CBC_UPT', 'CBC_TOTAL_PT') else None
+            if cbc is not None:
+                ptype = 'characteristic'              # BCs.py:155,167
+            self.patches.append(dict(name=name, start=b['startFace'], n=b['nFaces'],
+                                     cstart=b['startFace'] - self.nIF + self.nIC, type=ptype,
+                                     nbr=m.boundary[b['neighbourPatch']]['startFace'] if b['type'] in CYCLIC_TYPES else None,
+                                     spec=spec, cbc=cbc))
+        # ghost gradient source (default boundaries of the gradient fields: copy
+        # owner, or the paired owner on cyclic patches), density.py:376-385
+        gsrc = torch.arange(self.nC)
+        for p in self.patches:
+            if p['n'] == 0:
+                continue
+            if m.boundary[p['name']]['type'] in CYCLIC_TYPES:
+                gsrc[p['cstart']:p['cstart'] + p['n']] = self.owner[p['nbr']:p['nbr'] + p['n']]
+            else:
+                gsrc[p['cstart']:p['cstart'] + p['n']] = self.owner[p['start']:p['start'] + p['n']]
+        self.grad_src = gsrc
+
+    # --------------------------------------------------------- primitives --
+    def primitive(self, rho, rhoU, rhoE):
+        """adFVM/density.py:162-171."""
+        U = rhoU / rho
+        E = rhoE / rho
+        e = E - 0.5 * (U * U).sum(-1, keepdim=True)
+        p = (self.gamma - 1) * rho * e
+        T = e * (1. / self.Cv)
+        return U, T, p
+
+    def conservative(self, U, T, p):
+        """adFVM/density.py:173-181."""
+        e = self.Cv * T
+        rho = p / (e * (self.gamma - 1))
+        rhoE = rho * (e + 0.5 * (U * U).sum(-1, keepdim=True))
+        rhoU = U * rho
+        return rho, rhoU, rhoE
+
+    # ------------------------------------------------------------ ghosts --
+    def ghosts(self, U, T, p, ghostInit=None):
+        """Fill ghost rows: per field, patches in sorted order (field.py:148-156,
+        BCs.py:86-133), then characteristic patches (density.py:333-337,
+        BCs.py:152-184).  U,T,p: internal (nIC,k); returns (nC,k)."""
+        nG = self.nC - self.nIC
+        if ghostInit is None:
+            ghostInit = [torch.zeros(nG, 3, dtype=U.dtype), torch.zeros(nG, 1, dtype=U.dtype),
+                         torch.zeros(nG, 1, dtype=U.dtype)]
+        fields = {'U': torch.cat([U, ghostInit[0]]), 'T': torch.cat([T, ghostInit[1]]),
+                  'p': torch.cat([p, ghostInit[2]])}
+        n_all = self.t['normals']
+        for name in ('U', 'T', 'p'):
+            phi = fields[name]
+            parts = [phi[:self.nIC]]
+            cur = self.nIC
+            gh = phi[self.nIC:].clone()
+            rows = {}
+            for pt in self.patches:
+                if pt['n'] == 0:
+                    continue
+                s, n, cs = pt['start'], pt['n'], pt['cstart']
+                kind = pt['spec'][name]['type']
+                own = self.owner[s:s + n]
+                if kind in CYCLIC_TYPES:
+                    val = phi[self.owner[pt['nbr']:pt['nbr'] + n]]
+                elif kind in COPY_TYPES:
+                    val = phi[own]
+                elif kind in SYMMETRY_TYPES:
+                    val = phi[own]
+                    if name == 'U':
+                        nn = n_all[s:s + n]
+                        val = val - (val * nn).sum(-1, keepdim=True) * nn
+                elif kind == 'fixedValue':
+                    val = bvalue(pt['spec'][name]['value'], n, 3 if name == 'U' else 1).to(phi.dtype) * 1
+                elif kind in ('calculated', 'CBC_UPT', 'CBC_TOTAL_PT'):
+                    continue
+                else:
+                    raise NotImplementedError(kind)
+                rows[cs - self.nIC] = (n, val)
+            pieces, g = [], 0
+            for off in sorted(rows):
+                if off > g:
+                    pieces.append(gh[g:off])
+                pieces.append(rows[off][1])
+                g = off + rows[off][0]
+            if g < nG:
+                pieces.append(gh[g:])
+            fields[name] = torch.cat(parts + pieces)
+        # characteristic BCs
+        for pt in self.patches:
+            if pt['cbc'] is None or pt['n'] == 0:
+                continue
+            s, n, cs = pt['start'], pt['n'], pt['cstart']
+            sp = pt['spec']['p']
+            if pt['cbc'] == 'CBC_UPT':
+                Ub = bvalue(sp['U0'], n, 3) * 1
+                Tb = bvalue(sp['T0'], n, 1) * 1
+                pb = bvalue(sp['p0'], n, 1) * 1
+            else:
+                own = self.owner[s:s + n]
+                Uo, To = fields['U'][own], fields['T'][own]
+                d = bvalue(sp['direction'], n, 3) if 'direction' in sp else n_all[s:s + n]
+                Tt = bvalue(sp['Tt'], n, 1)
+                ptot = bvalue(sp['pt'], n, 1)
+                Un = (Uo * d).sum(-1, keepdim=True)
+                Ub = Un * d
+                Tb = Tt - 0.5 * Un * Un / self.Cp
+                pb = ptot * (To / Tt) ** (self.gamma / (self.gamma - 1))
+            for name, val in (('U', Ub), ('T', Tb), ('p', pb)):
+                phi = fields[name]
+                fields[name] = torch.cat([phi[:cs], val.to(phi.dtype), phi[cs + n:]])
+        return fields['U'], fields['T'], fields['p']
+
+    # -------------------------------------------------------- operators --
+    def grad_cell(self, phi):
+        """Gauss gradient over the 6 faces, adFVM/op.py:45-63.  phi (nC,k) ->
+        (nIC,k,3)."""
+        S = self.t['areas'][self.cellFaces]                 # (nIC,6)
+        N = self.t['normals'][self.cellFaces]               # (nIC,6,3)
+        w = self.t['weights'][self.cellFaces]
+        O = self.cellOwner.to(phi.dtype)
+        N = 2 * N * O[..., None] - N
+        w = w + O - 2 * w * O
+        phiP = phi[self.cellNeighbours]                     # (nIC,6,k)
+        phic = phi[:self.nIC]
+        g = 0
+        for i in range(6):
+            phiF = phic * (-w[:, i:i + 1] + 1) + phiP[:, i] * w[:, i:i + 1]
+            g = g + phiF[:, :, None] * (S[:, i, None, None] * N[:, i, None, :])
+        return g / self.t['volumes'][:, None, None]
+
+    def central(self, phi):
+        """adFVM/interp.py:15-18 (face values on all faces)."""
+        f = self.t['weights'][:, None]
+        return phi[self.owner] * f + phi[self.neighbour] * (-f + 1)
+
+    def second_order(self, phi, grad, swap, faces=None):
+        """adFVM/interp.py:20-28: phi_C + (phi_D-phi_C)*lw + qw.grad(phi_C)."""
+        faces = slice(None) if faces is None else faces
+        p, n = self.owner[faces], self.neighbour[faces]
+        gsrc = self.grad_src
+        if swap:
+            p, n = n, p
+        phiC, phiD = phi[p], phi[n]
+        lw = self.t['linearWeights'][faces][:, swap:swap + 1]
+        qw = self.t['quadraticWeights'][faces][:, swap]        # (nf,3)
+        gC = grad[gsrc[p]]                                       # (nf,k,3)
+        out = phiC + (phiD - phiC) * lw + (gC * qw[:, None, :]).sum(-1)
+        if self.limiter:
+            # optional clamp between the two cell values (not in the reference)
+            out = torch.minimum(torch.maximum(out, torch.minimum(phiC, phiD)), torch.maximum(phiC, phiD))
+        return out
+
+    # --------------------------------------------------- Riemann solvers --
+    def stabilise(self, x):
+        """eps.stabilise(SMALL), riemann.py:52; taken as x + sign(x)*SMALL
+        (x >= 0 here).  adpy is absent, so this is parity unpinned."""
+        return torch.where(x < 0, x - self.small, x + self.small)
+
+    def euler_roe(self, pL, pR, TL, TR, UL, UR, rhoL, rhoR, rhoUL, rhoUR, rhoEL, rhoER, n):
+        """adFVM/riemann.py:21-70."""
+        g = self.gamma
+        dot = lambda a, b: (a * b).sum(-1, keepdim=True)
+        rhoUnL, rhoUnR = rhoL * dot(UL, n), rhoR * dot(UR, n)
+        hL = g * pL / ((g - 1) * rhoL) + 0.5 * dot(UL, UL)
+        hR = g * pR / ((g - 1) * rhoR) + 0.5 * dot(UR, UR)
+        Frho = 0.5 * (rhoUnL + rhoUnR)
+        FrhoU = 0.5 * (rhoUnL * UL + rhoUnR * UR + (pL + pR) * n)
+        FrhoE = 0.5 * (rhoUnL * hL + rhoUnR * hR)
+        sL, sR = rhoL.sqrt(), rhoR.sqrt()
+        div = sL + sR
+        UF = (UL * sL + UR * sR) / div
+        hF = (hL * sL + hR * sR) / div
+        qF = 0.5 * dot(UF, UF)
+        a2F = (g - 1) * (hF - qF)
+        aF = a2F.sqrt()
+        UnF = dot(UF, n)
+        drho = rhoR - rhoL
+        drhoU = rhoR * UR - rhoL * UL
+        drhoE = (hR * rhoR - pR) - (hL * rhoL - pL)
+        lam1, lam2, lam3 = UnF.abs(), (UnF + aF).abs(), (UnF - aF).abs()
+        eps = 0.5 * (rhoUnL / rhoL - rhoUnR / rhoR).abs()
+        eps = eps + 0.5 * ((g * pL / rhoL).sqrt() - (g * pR / rhoR).sqrt()).abs()
+        eps = self.stabilise(eps)
+        fix = lambda lam: torch.where(lam < 2. * eps, .25 * lam * lam / eps + eps, lam)
+        lam1, lam2, lam3 = fix(lam1), fix(lam2), fix(lam3)
+        abv1 = 0.5 * (lam2 + lam3)
+        abv2 = 0.5 * (lam2 - lam3)
+        abv3 = abv1 - lam1
+        abv4 = (g - 1) * (qF * drho - dot(UF, drhoU) + drhoE)
+        abv5 = UnF * drho - dot(drhoU, n)
+        abv6 = abv3 * abv4 / a2F - abv2 * abv5 / aF
+        abv7 = abv3 * abv5 - abv2 * abv4 / aF
+        Frho = Frho - 0.5 * (lam1 * drho + abv6)
+        FrhoU = FrhoU - 0.5 * (lam1 * drhoU + UF * abv6 - abv7 * n)
+        FrhoE = FrhoE - 0.5 * (lam1 * drhoE + hF * abv6 - UnF * abv7)
+        return Frho, FrhoU, FrhoE
+
+    def euler_lf(self, pL, pR, TL, TR, UL, UR, rhoL, rhoR, rhoUL, rhoUR, rhoEL, rhoER, n):
+        """adFVM/riemann.py:6-18."""
+        g = self.gamma
+        UnL, UnR = (UL * n).sum(-1, keepdim=True), (UR * n).sum(-1, keepdim=True)
+        cL = (g * pL / rhoL).sqrt()
+        cR = (g * pR / rhoR).sqrt()
+        aF = torch.maximum(UnL.abs() + cL, UnR.abs() + cR)
+        Frho = 0.5 * (rhoL * UnL + rhoR * UnR) - 0.5 * aF * (rhoR - rhoL)
+        FrhoU = 0.5 * (rhoUL * UnL + rhoUR * UnR + (pL + pR) * n) - 0.5 * aF * (rhoUR - rhoUL)
+        FrhoE = 0.5 * ((rhoEL + pL) * UnL + (rhoER + pR) * UnR) - 0.5 * aF * (rhoER - rhoEL)
+        return Frho, FrhoU, FrhoE
+
+    def riemann_fn(self, name):
+        return self.euler_roe if name == 'roe' else self.euler_lf
+
+    # ------------------------------------------------------------ fluxes --
+    def viscous_flux(self, TL, TR, UL, UR, TF, UF, gradTF, gradUF, faces):
+        """adFVM/density.py:192-222 with op.snGrad (op.py:66-67)."""
+        mu = self.mu(TF)
+        kappa = mu * (self.Cp / self.Pr)                     # density.py:46
+        D = self.t['deltasUnit'][faces]
+        N = self.t['normals'][faces]
+        dl = self.t['deltas'][faces][:, None]
+        gT = gradTF[:, 0]                                    # (nf,3)
+        gT = gT + ((TR - TL) / dl) * D - (gT * D).sum(-1, keepdim=True) * D
+        snU = (UR - UL) / dl                                 # (nf,3)
+        gU = gradUF + snU[:, :, None] * D[:, None, :] - (gradUF * D[:, None, :]).sum(-1)[:, :, None] * D[:, None, :]
+        qF = kappa * (gT * N).sum(-1, keepdim=True)
+        tmp2 = ((gU + gU.transpose(1, 2)) * N[:, None, :]).sum(-1)
+        tmp3 = (gU[:, 0, 0] + gU[:, 1, 1] + gU[:, 2, 2])[:, None]
+        sigma = mu * (tmp2 - 2. / 3 * tmp3 * N)
+        return -sigma, -(qF + (sigma * UF).sum(-1, keepdim=True))
+
+    def face_flux(self, U, T, p, gU, gT, gp, faces, characteristic=False, boundary=False):
+        """RCF.flux (density.py:253-305) or RCF.boundaryFlux (:307-331) on a
+        face range.  Returns (Frho, FrhoU, FrhoE, maxaF) per face."""
+        P, N = self.owner[faces], self.neighbour[faces]
+        n = self.t['normals'][faces]
+        gs = self.grad_src
+        TL, TR = T[P], T[N]
+        UL, UR = U[P], U[N]
+        if boundary:
+            pR = p[N]
+            gUR, gTR = gU[gs[N]], gT[gs[N]]
+            rhoR, rhoUR, rhoER = self.conservative(UR, TR, pR)
+            Un = (UR * n).sum(-1, keepdim=True)
+            Frho = rhoR * Un
+            FrhoU = rhoUR * Un + pR * n
+            FrhoE = (rhoER + pR) * Un
+            vU, vE = self.viscous_flux(TL, TR, UL, UR, TR, UR, gTR, gUR, faces)
+            aF = (self.Cp * TR * (self.gamma - 1)).sqrt()
+            maxaF = (UR * n).sum(-1, keepdim=True).abs() + aF
+            return Frho, FrhoU + vU, FrhoE + vE, maxaF
+        ULF = self.second_order(U, gU, 0, faces)
+        TLF = self.second_order(T, gT, 0, faces)
+        pLF = self.second_order(p, gp, 0, faces)
+        rhoLF, rhoULF, rhoELF = self.conservative(ULF, TLF, pLF)
+        if characteristic:
+            URF, TRF, pRF = UR, TR, p[N]
+            rhoRF, rhoURF, rhoERF = self.conservative(URF, TRF, pRF)
+            Frho, FrhoU, FrhoE = self.riemann_fn(self.boundaryRiemann)(
+                pLF, pRF, TLF, TRF, ULF, URF, rhoLF, rhoRF, rhoULF, rhoURF, rhoELF, rhoERF, n)
+            TF, UF = TR, UR
+            gTF, gUF = gT[gs[N]], gU[gs[N]]
+        else:
+            URF = self.second_order(U, gU, 1, faces)
+            TRF = self.second_order(T, gT, 1, faces)
+            pRF = self.second_order(p, gp, 1, faces)
+            rhoRF, rhoURF, rhoERF = self.conservative(URF, TRF, pRF)
+            Frho, FrhoU, FrhoE = self.riemann_fn(self.riemann)(
+                pLF, pRF, TLF, TRF, ULF, URF, rhoLF, rhoRF, rhoULF, rhoURF, rhoELF, rhoERF, n)
+            UF = 0.5 * (ULF + URF)
+            TF = 0.5 * (TLF + TRF)
+            gTF = 0.5 * (gT[gs[P]] + gT[gs[N]])
+            gUF = 0.5 * (gU[gs[P]] + gU[gs[N]])
+        vU, vE = self.viscous_flux(TL, TR, UL, UR, TF, UF, gTF, gUF, faces)
+        aF = (self.Cp * TF * (self.gamma - 1)).sqrt()
+        maxaF = (UF * n).sum(-1, keepdim=True).abs() + aF
+        return Frho, FrhoU + vU, FrhoE + vE, maxaF
+
+    def all_face_fluxes(self, U, T, p, gU, gT, gp):
+        """Per-face flux for every face: internal faces (density.py:390),
+        then each sorted patch by its type (:391-400)."""
+        nF = self.nF
+        out = [None] * 4
+        parts = [(slice(0, self.nIF), self.face_flux(U, T, p, gU, gT, gp, slice(0, self.nIF)))]
+        for pt in self.patches:
+            if pt['n'] == 0:
+                continue
+            sl = slice(pt['start'], pt['start'] + pt['n'])
+            if pt['type'] in CYCLIC_TYPES:
+                r = self.face_flux(U, T, p, gU, gT, gp, sl)
+            elif pt['type'] == 'characteristic':
+                r = self.face_flux(U, T, p, gU, gT, gp, sl, characteristic=True)
+            else:
+                r = self.face_flux(U, T, p, gU, gT, gp, sl, boundary=True)
+            parts.append((sl, r))
+        parts.sort(key=lambda x: x[0].start)
+        for i in range(4):
+            out[i] = torch.cat([r[i] for _, r in parts])
+        return out
+
+    def accumulate(self, faceVals, absolute=False):
+        """div / absDiv (adFVM/op.py:12-29): owner += F*(A/V_P), neighbour
+        += -F*(A/V_N) (or +, absolute), summed over the cell's cellFaces
+        slots in order.  faceVals (nF,k) -> (nIC,k)."""
+        A = self.t['areas']
+        V = self.t['volumes']
+        wc = (A[self.cellFaces] / V[:, None])                       # (nIC,6)
+        sign = self.cellOwner.to(faceVals.dtype) * 2 - 1
+        if absolute:
+            sign = torch.ones_like(sign)
+        acc = 0
+        for j in range(6):
+            f = self.cellFaces[:, j]
+            acc = acc + faceVals[f] * (sign[:, j] * wc[:, j])[:, None]
+        return acc
+
+    def equation(self, rho, rhoU, rhoE, ghostInit=None, with_fields=False):
+        """RCF.equation (density.py:339-414): (drho, drhoU, drhoE, dtc_max)."""
+        U, T, p = self.primitive(rho, rhoU, rhoE)
+        U, T, p = self.ghosts(U, T, p, ghostInit)
+        gU = self.grad_cell(U)
+        gT = self.grad_cell(T)
+        gp = self.grad_cell(p)
+        # gradient ghost rows: copies of owner / paired-owner rows (grad_src)
+        Fr, FrU, FrE, lam = self.all_face_fluxes(U, T, p, gU, gT, gp)
+        drho = self.accumulate(Fr)
+        drhoU = self.accumulate(FrU)
+        drhoE = self.accumulate(FrE)
+        dtc = self.accumulate(lam, absolute=True)
+        out = (drho, drhoU, drhoE, dtc)
+        if with_fields:
+            return out, (U, T, p, gU, gT, gp)
+        return out
+
+    # ---------------------------------------------------------- objectives --
+    def objective_value(self, rho, rhoU, rhoE, U, T, p):
+        if self.objective is None:
+            return torch.zeros((), dtype=rho.dtype)
+        kind = self.objective[0]
+        if kind == 'kinetic_energy':
+            Uc = rhoU / rho
+            return (0.5 * rho[:, 0] * (Uc * Uc).sum(-1) * self.t['volumes']).sum()
+        if kind == 'drag':
+            pt = [q for q in self.patches if q['name'] == self.objective[1]][0]
+            s, n = pt['start'], pt['n']
+            faces = slice(s, s + n)
+            N, P = self.neighbour[faces], self.owner[faces]
+            U0, U0i = U[N][:, 0:1], U[P][:, 0:1]
+            p0, T0 = p[N], T[N]
+            nx = self.t['normals'][faces][:, 0:1]
+            mungUx = self.mu(T0) * (U0 - U0i) / self.t['deltas'][faces][:, None]
+            return ((p0 * nx - mungUx) * self.t['areas'][faces][:, None]).sum()
+        raise NotImplementedError(kind)
+
+    # --------------------------------------------------------------- step --
+    def coefficients(self):
+        if self.integrator == 'SSPRK':                          # timestep.py:17-25
+            alpha = [[1, 0, 0], [3. / 4, 1. / 4, 0], [1. / 3, 0, 2. / 3]]
+            beta = [1, 1. / 4, 2. / 3]
+        else:                                                   # timestep.py:9-13
+            alpha, beta = [[1.]], [1.]
+        return alpha, beta
+
+    def step(self, rho, rhoU, rhoE, dt, source=None, ghostInit=None):
+        """One time step (timestep.py:27-55): returns (rho', rhoU', rhoE',
+        dtc, obj) with dtc and obj taken on the stage-1 state
+        (density.py:412-413)."""
+        alpha, beta = self.coefficients()
+        fields = [[rho, rhoU, rhoE]]
+        dtc = obj = None
+        for i in range(len(beta)):
+            (d0, d1, d2, dtcc) = self.equation(*fields[i], ghostInit=ghostInit)
+            if i == 1 or len(beta) == 1:
+                dtc = dtcc.max()
+                U, T, p = self.primitive(*fields[i])
+                U, T, p = self.ghosts(U, T, p, ghostInit)
+                obj = self.objective_value(*fields[i], U, T, p)
+            L = [d0, d1, d2]
+            S = source if source is not None else [0., 0., 0.]
+            new = []
+            for k in range(3):
+                acc = 0
+                for j in range(i + 1):
+                    if alpha[i][j] != 0:
+                        acc = acc + alpha[i][j] * fields[j][k]
+                acc = acc + -beta[i] * (L[k] - S[k]) * dt
+                new.append(acc)
+            fields.append(new)
+        return fields[-1][0], fields[-1][1], fields[-1][2], dtc, obj
+
+
+def to_state(rho, rhoU, rhoE):
+    """SoA state u[5][n] (numpy) from reference-layout arrays."""
+    return np.concatenate([np.asarray(rho).reshape(1, -1), np.asarray(rhoU).T, np.asarray(rhoE).reshape(1, -1)])
+
+
+def from_state(u):
+    u = np.asarray(u)
+    return u[0:1].T.copy(), u[1:4].T.copy(), u[4:5].T.copy()

@@ -1,0 +1,168 @@
+"""Shared case definitions for the oracle and the HIP path: the same mesh,
+boundary spec, physics and initial state feed both sides of every parity
+test."""
+import numpy as np
+
+from adfvm_amd import meshgen, _lib
+from adfvm_amd.mesh import Mesh
+
+GAMMA, CP, PR = 1.4, 1004.5, 0.7
+R = CP - CP / GAMMA
+
+
+class CaseDef(object):
+    def __init__(self, md, bcs, mu0, riemann='roe', boundaryRiemann='roe', objective=None, sutherland=False,
+                 limiter=False):
+        self.md = md
+        self.limiter = limiter
+        self.bcs = bcs
+        self.mu0 = mu0
+        self.riemann = riemann
+        self.boundaryRiemann = boundaryRiemann
+        self.objective = objective
+        self.sutherland = sutherland
+        self._mesh = None
+
+    @property
+    def mesh(self):
+        if self._mesh is None:
+            self._mesh = Mesh.from_data(self.md)
+        return self._mesh
+
+    def physics(self, precision):
+        small = 1e-30 if np.dtype(precision) == np.float64 else 1e-9
+        mu = (_lib.MU_SUTHERLAND, 0., 1.4792e-06, 116.) if self.sutherland else (_lib.MU_CONSTANT, self.mu0, 0., 0.)
+        ph = dict(gamma=GAMMA, Cp=CP, Pr=PR, mu=mu,
+                  riemann=_lib.RIEMANN_ROE if self.riemann == 'roe' else _lib.RIEMANN_LF,
+                  boundaryRiemann=_lib.RIEMANN_ROE if self.boundaryRiemann == 'roe' else _lib.RIEMANN_LF,
+                  small=small, integrator=_lib.RK_SSPRK3, limiter=int(self.limiter))
+        if self.objective:
+            ph['objective'] = {'kinetic_energy': _lib.OBJ_KINETIC_ENERGY, 'drag': _lib.OBJ_DRAG}[self.objective[0]]
+            if self.objective[0] == 'drag':
+                ph['objectivePatch'] = self.objective[1]
+        return ph
+
+    def context(self, precision):
+        from adfvm_amd.context import Context
+        return Context(self.mesh, self.bcs, self.physics(precision), precision)
+
+    def oracle(self, small=1e-30):
+        from oracle import mesh_oracle, fvm_oracle
+        om = mesh_oracle.build(self.md)
+        if self.sutherland:
+            mu = None
+        else:
+            mu0 = self.mu0
+            mu = lambda T: T * 0 + mu0
+        return fvm_oracle.Case(om, self.bcs, GAMMA, CP, PR, mu, self.riemann, self.boundaryRiemann, small,
+                               self.objective, limiter=self.limiter)
+
+
+def conservative(U, T, p):
+    Cv = CP / GAMMA
+    e = Cv * T
+    rho = p / (e * (GAMMA - 1))
+    rhoE = rho * (e + 0.5 * (U * U).sum(-1, keepdims=True))
+    return rho, U * rho, rhoE
+
+
+def state_from_prims(U, T, p):
+    rho, rhoU, rhoE = conservative(U, T, p)
+    return np.concatenate([rho.T, rhoU.T, rhoE.T]).astype(np.float64)
+
+
+def tgv(n, seed=0, rcm=True, objective=('kinetic_energy',)):
+    """Taylor-Green vortex on a periodic [0,2pi]^3 box, Mach 0.1, rho0 = 1.2,
+    Re = 1600 (mu = rho0 U0 L/Re, L = 1), T0 = 300 K."""
+    md = meshgen.permute_and_rcm(meshgen.periodic_box(n), seed=seed, rcm=rcm)
+    rho0, T0 = 1.2, 300.
+    c0 = np.sqrt(GAMMA * R * T0)
+    U0 = 0.1 * c0
+    mu = rho0 * U0 * 1.0 / 1600.
+    return CaseDef(md, {}, mu, objective=objective), dict(rho0=rho0, T0=T0, U0=U0)
+
+
+def tgv_state(case, params):
+    m = case.mesh
+    x, y, z = [m.cellCentres[:m.nInternalCells, i:i + 1] for i in range(3)]
+    U0, rho0, T0 = params['U0'], params['rho0'], params['T0']
+    p0 = rho0 * R * T0
+    U = np.concatenate([U0 * np.sin(x) * np.cos(y) * np.cos(z), -U0 * np.cos(x) * np.sin(y) * np.cos(z), 0 * x], 1)
+    p = p0 + rho0 * U0 ** 2 / 16 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2)
+    T = T0 + 0 * x
+    return state_from_prims(U, T, p)
+
+
+def sod(nx=100, dy=None):
+    """Shock tube on [0,1]: left p=1e5, T=348.4; right p=1e4, T=278.7."""
+    md = meshgen.shock_tube(nx, 1.0, dy)
+    return CaseDef(md, {}, 0.0, limiter=True)
+
+
+def sod_state(case, split=0.5):
+    m = case.mesh
+    x = m.cellCentres[:m.nInternalCells, 0:1]
+    left = x < split
+    p = np.where(left, 1e5, 1e4)
+    T = np.where(left, 348.4, 278.7)
+    U = np.zeros((len(x), 3))
+    return state_from_prims(U, T, p)
+
+
+def mixed_box(seed=0, cbc='CBC_UPT', riemann='roe', boundaryRiemann='roe', sutherland=False):
+    """Every ghost rule at once on a small random-renumbered box: fixedValue
+    inlet, characteristic outlet, slip and no-slip walls, empty sides."""
+    patches = {'x0': ('inlet', 'patch'), 'x1': ('outlet', 'patch'),
+               'y0': ('bottom', 'symmetryPlane'), 'y1': ('top', 'patch'),
+               'z0': ('sides', 'empty'), 'z1': ('sides', 'empty')}
+    md = meshgen.permute_and_rcm(meshgen.box((6, 5, 3), (1.2, 1.0, 0.6), patches=patches), seed=seed)
+    rng = np.random.default_rng(seed)
+    n_in, n_out, n_top = 5 * 3, 5 * 3, 6 * 3
+    bcs = {
+        'inlet': {'U': {'type': 'fixedValue', 'value': np.array([30., 2., 0.])},
+                  'T': {'type': 'fixedValue', 'value': 300 + rng.random((n_in, 1))},
+                  'p': {'type': 'zeroGradient'}},
+        'top': {'U': {'type': 'fixedValue', 'value': np.zeros(3)},
+                'T': {'type': 'zeroGradient'},
+                'p': {'type': 'zeroGradient'}},
+        'bottom': {'U': {'type': 'slip'}, 'T': {'type': 'symmetryPlane'}, 'p': {'type': 'symmetryPlane'}},
+    }
+    if cbc == 'CBC_UPT':
+        bcs['outlet'] = {'U': {'type': 'CBC_UPT'}, 'T': {'type': 'CBC_UPT'},
+                         'p': {'type': 'CBC_UPT', 'U0': np.array([25., 0., 0.]), 'T0': np.array([299.]),
+                               'p0': 1e5 + 10 * rng.random((n_out, 1))}}
+    elif cbc == 'CBC_TOTAL_PT':
+        bcs['outlet'] = {'U': {'type': 'CBC_TOTAL_PT'}, 'T': {'type': 'CBC_TOTAL_PT'},
+                         'p': {'type': 'CBC_TOTAL_PT', 'Tt': np.array([301.]), 'pt': np.array([1.02e5])}}
+    else:
+        bcs['outlet'] = {'U': {'type': 'zeroGradient'}, 'T': {'type': 'calculated', 'value': np.array([299.])},
+                         'p': {'type': 'calculated', 'value': np.array([1.0e5])}}
+    return CaseDef(md, bcs, 1.8e-5, riemann=riemann, boundaryRiemann=boundaryRiemann, sutherland=sutherland)
+
+
+def smooth_state(case, seed=0, U=(25., 3., 1.), amp=0.05):
+    m = case.mesh
+    rng = np.random.default_rng(seed)
+    x = m.cellCentres[:m.nInternalCells]
+    k = rng.normal(size=(3, 3)) * 2
+    ph = rng.random(3) * 6
+    s = np.sin(x @ k + ph)
+    Uf = np.array(U)[None, :] * (1 + amp * s)
+    T = 300 * (1 + amp * s[:, 1:2])
+    p = 1e5 * (1 + amp * s[:, 2:3])
+    return state_from_prims(Uf, T, p)
+
+
+def ghost_init(case):
+    """Ghost values the 'calculated' patches hold (their file values)."""
+    m = case.mesh
+    nG = m.nFaces - m.nInternalFaces
+    U = np.zeros((nG, 3)); T = np.zeros((nG, 1)); p = np.zeros((nG, 1))
+    for name, spec in case.bcs.items():
+        b = m.boundary[name]
+        s = b['startFace'] - m.nInternalFaces
+        n = b['nFaces']
+        for f, arr, k in (('U', U, 3), ('T', T, 1), ('p', p, 1)):
+            if spec.get(f, {}).get('type') == 'calculated':
+                arr[s:s + n] = np.broadcast_to(np.asarray(spec[f]['value'], float).reshape(-1, k), (n, k))
+    return U, T, p

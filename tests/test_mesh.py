@@ -1,0 +1,138 @@
+"""Mesh index construction and geometry: the C++ builder (libadfvm, host
+code) against the oracle restatement of cmesh.cpp / mesh.py, bit for bit,
+plus closed forms and brute force that pin the oracle itself."""
+import numpy as np
+import pytest
+
+from adfvm_amd import meshgen
+from adfvm_amd.mesh import Mesh
+from oracle import mesh_oracle as mo
+
+INT_FIELDS = ['cellFaces', 'cells', 'neighbour', 'cellOwner', 'cellNeighbours', 'cellNeighboursMatOp']
+FP_FIELDS = ['normals', 'faceCentres', 'areas', 'cellCentres', 'volumes', 'deltas', 'deltasUnit',
+             'weights', 'linearWeights', 'quadraticWeights', 'volumesL', 'volumesR']
+
+MESHES = {
+    'box': lambda: meshgen.box((3, 2, 2), (1.5, 1., 0.5)),
+    'periodic': lambda: meshgen.periodic_box(4),
+    'tube': lambda: meshgen.shock_tube(10),
+    'ogrid': lambda: meshgen.o_grid(16, 4),
+    'periodic_perm_rcm': lambda: meshgen.permute_and_rcm(meshgen.periodic_box(6), seed=1),
+    'ogrid_perm_rcm': lambda: meshgen.permute_and_rcm(meshgen.o_grid(12, 5), seed=3),
+    'graded_box': lambda: meshgen.box((5, 3, 2), x_coords=np.array([0, .1, .3, .6, 1.0, 1.5])),
+}
+
+
+@pytest.mark.parametrize('name', sorted(MESHES))
+def test_builder_matches_oracle_bitwise(lib, name):
+    md = MESHES[name]()
+    m = Mesh.from_data(md)
+    o = mo.build(md, with_cells=True)
+    for k in INT_FIELDS:
+        assert np.array_equal(getattr(m, k), getattr(o, k)), k
+    for k in FP_FIELDS:
+        a, b = getattr(m, k).reshape(-1), getattr(o, k).reshape(-1)
+        assert np.array_equal(a, b), (k, np.abs(a - b).max())
+    assert m.nLocalCells == o.nLocalCells
+    assert m.sortedPatches == o.sortedPatches
+    for p in m.boundary:
+        assert m.boundary[p]['cellStartFace'] == o.boundary[p]['cellStartFace']
+
+
+def test_cell_faces_fill_order_vs_literal_loop():
+    """The oracle's vectorised cellFaces equals the literal two-loop fill of
+    cmesh.cpp:9-19."""
+    md = meshgen.permute_and_rcm(meshgen.periodic_box((4, 3, 5)), seed=7)
+    o = mo.build(md)
+    assert np.array_equal(o.cellFaces, mo.cell_faces_loop(o.owner, o.neighbour[:o.nInternalFaces], o.nInternalCells))
+
+
+def test_brute_force_2x1x1():
+    """Hand-enumerated 2x1x1 box: faces 0 (internal), patches after."""
+    md = meshgen.box((2, 1, 1))
+    o = mo.build(md)
+    assert o.nInternalFaces == 1 and o.nFaces == 11 and o.nInternalCells == 2
+    assert list(o.cellFaces[0]) == [0] + [f for f in range(1, 11) if md.owner[f] == 0]
+    assert list(o.cellFaces[1])[:5] == [f for f in range(1, 11) if md.owner[f] == 1]
+    assert o.cellFaces[1][5] == 0
+    # ghost numbering: ghost of boundary face f is nIC + f - nIF
+    assert np.array_equal(o.neighbour[1:], 2 + np.arange(10))
+    assert np.array_equal(o.cellOwner[1], [1, 1, 1, 1, 1, 0])
+    assert o.cellNeighbours[1][5] == 0 and o.cellNeighbours[0][0] == 1
+
+
+def test_closed_form_cartesian_geometry():
+    """Uniform box: n = +-axis, A = dy dz, V = dx dy dz, internal w = 1/2,
+    lw = w1/3 = 1/6, qw = 2/3 (F-P) (its 1/3 (pF + w1 delta) term vanishes
+    at w1 = 1/2), deltas = dx."""
+    dx, dy, dz = 0.5, 0.25, 0.125
+    md = meshgen.box((4, 4, 4), (4 * dx, 4 * dy, 4 * dz))
+    o = mo.build(md)
+    nIF = o.nInternalFaces
+    assert np.allclose(o.volumes, dx * dy * dz, rtol=1e-14, atol=0)
+    n = o.normals[:nIF]
+    assert np.allclose(np.abs(n).max(1), 1) and np.allclose(np.abs(n).sum(1), 1)
+    ax = np.abs(n).argmax(1)
+    assert np.allclose(o.areas[:nIF], np.array([dy * dz, dx * dz, dx * dy])[ax], rtol=1e-14)
+    assert np.allclose(o.deltas[:nIF], np.array([dx, dy, dz])[ax], rtol=1e-14)
+    assert np.allclose(o.weights[:nIF], 0.5, rtol=1e-14)
+    assert np.allclose(o.linearWeights[:nIF], 1. / 6, rtol=1e-13)
+    F = o.faceCentres[:nIF]
+    P = o.cellCentres[o.owner[:nIF]]
+    N = o.cellCentres[o.neighbour[:nIF]]
+    assert np.allclose(o.quadraticWeights[:nIF, 0], 2. / 3 * (F - P), atol=1e-14)
+    assert np.allclose(o.quadraticWeights[:nIF, 1], 2. / 3 * (F - N), atol=1e-14)
+    # boundary faces: ghost at the face centre -> weight 0 (owner side 1 in gradCell)
+    assert np.allclose(o.weights[nIF:], 0)
+    assert np.allclose(o.cellCentres[o.nInternalCells:], o.faceCentres[nIF:])
+
+
+def test_closed_form_ogrid_volume():
+    """O-grid cell volumes are exact for the polygonal annulus: the sum
+    equals (area between two regular n-gons) * dz."""
+    nt, nr = 32, 6
+    md = meshgen.o_grid(nt, nr, r0=0.5, r1=3.0, ratio=1.1)
+    o = mo.build(md)
+    dz = 2 * np.pi * 0.5 / nt
+    poly = lambda r: 0.5 * nt * r * r * np.sin(2 * np.pi / nt)
+    assert np.isclose(o.volumes.sum(), (poly(3.0) - poly(0.5)) * dz, rtol=1e-13)
+    # the seam is internal: every cell has exactly 6 faces and no cyclic patch
+    assert o.nInternalFaces == nt * nr + nt * (nr - 1)
+
+
+def test_cyclic_ghost_centres_are_translated_partners():
+    """createGhostCells (mesh.py:774-783): ghost centre = partner owner's
+    centre + (faceCentre[start] - faceCentre[nbrStart])."""
+    L = 2 * np.pi
+    md = meshgen.permute_and_rcm(meshgen.periodic_box(5, L), seed=2)
+    o = mo.build(md)
+    b = o.boundary['patch0_half0']
+    s, n = b['startFace'], b['nFaces']
+    g = o.cellCentres[o.nInternalCells + s - o.nInternalFaces: o.nInternalCells + s - o.nInternalFaces + n]
+    partner = o.cellCentres[o.owner[o.boundary['patch0_half1']['startFace']:][:n]]
+    assert np.allclose(g, partner - np.array([L, 0, 0]), atol=1e-12)
+    assert np.allclose(o.weights[s:s + n], 0.5, atol=1e-13)
+
+
+def test_rcm_matches_scipy_and_is_a_permutation():
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    md = meshgen.permute_and_rcm(meshgen.periodic_box(6), seed=5, rcm=False)
+    perm = meshgen.rcm_permutation(md)
+    order = reverse_cuthill_mckee(meshgen.cell_adjacency(md), symmetric_mode=True)
+    assert np.array_equal(np.sort(perm), np.arange(md.nCells))
+    assert np.array_equal(perm[order], np.arange(md.nCells))
+    out = meshgen.renumber_cells(md, perm)
+    A = meshgen.cell_adjacency(out).tocoo()
+    bw_after = np.abs(A.row - A.col).max()
+    B = meshgen.cell_adjacency(md).tocoo()
+    assert bw_after < np.abs(B.row - B.col).max()
+
+
+def test_reference_layout_lists():
+    m = Mesh.from_data(meshgen.periodic_box(3))
+    t = m.getTensor()
+    assert len(t) == 15 and t[0] is m.areas and t[-1] is m.cellOwner
+    s = m.getScalar()
+    assert len(s) == 8 + 3 * 6
+    assert m.areas.shape == (m.nFaces, 1) and m.quadraticWeights.shape == (m.nFaces, 2, 3)
+    assert m.faces.shape[1] == 5 and (m.faces[:, 0] == 4).all()
