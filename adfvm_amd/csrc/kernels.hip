@@ -460,8 +460,6 @@ __global__ void __launch_bounds__(BLOCK) axpby_kernel(long n, T a, const T* __re
     if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
-#include "adjoint_kernels.inc"
-
 // ----------------------------------------------------------- context -----
 
 struct adfvm_ctx {
@@ -914,3 +912,5 @@ extern "C" int adfvm_unpack_state(adfvm_ctx* c, const void* u, void* rho, void* 
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
 }
+
+#include "adjoint_kernels.inc"

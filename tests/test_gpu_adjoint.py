@@ -1,0 +1,137 @@
+"""HIP adjoint path against the oracle's autograd (the exact discrete
+adjoint of the oracle's step) and against central finite differences of the
+HIP primal itself.  FP64 tolerances are round-off level; FP32 tolerances are
+stated per test."""
+import numpy as np
+import pytest
+
+from tests import casefactory as cf
+from tests.test_gpu_primal import CASES, rel_l2
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+
+def oracle_residual_vjp(case, u, rbar):
+    oc = case.oracle()
+    x = torch.tensor(u, dtype=torch.float64, requires_grad=True)
+    gi = [torch.as_tensor(a, dtype=torch.float64) for a in cf.ghost_init(case)]
+    d0, d1, d2, _ = oc.equation(x[0:1].T, x[1:4].T, x[4:5].T, ghostInit=gi)
+    R = torch.cat([d0.T, d1.T, d2.T])
+    (g,) = torch.autograd.grad(R, x, torch.as_tensor(rbar))
+    return g.numpy()
+
+
+def oracle_step_vjp(case, u, dt, ua, obja, source=None):
+    oc = case.oracle()
+    x = torch.tensor(u, dtype=torch.float64, requires_grad=True)
+    S = None
+    if source is not None:
+        Sx = torch.tensor(source, dtype=torch.float64, requires_grad=True)
+        S = [Sx[0:1].T, Sx[1:4].T, Sx[4:5].T]
+    gi = [torch.as_tensor(a, dtype=torch.float64) for a in cf.ghost_init(case)]
+    r, rU, rE, dtc, obj = oc.step(x[0:1].T, x[1:4].T, x[4:5].T, dt, source=S, ghostInit=gi)
+    un = torch.cat([r.T, rU.T, rE.T])
+    L = (un * torch.as_tensor(ua)).sum() + obja * obj
+    inputs = [x] + ([Sx] if source is not None else [])
+    grads = torch.autograd.grad(L, inputs)
+    return [g.numpy() for g in grads]
+
+
+def scaled_rbar(case, u, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=u.shape) * np.abs(u).mean(1, keepdims=True) * 1e-3
+
+
+@pytest.mark.parametrize('name', sorted(CASES))
+def test_residual_vjp_matches_autograd(name):
+    case, u = CASES[name]()
+    rbar = scaled_rbar(case, u, 1)
+    ref = oracle_residual_vjp(case, u, rbar)
+    for prec, tol in ((np.float64, 1e-10), (np.float32, 2e-4)):
+        ctx = case.context(prec)
+        ud = torch.as_tensor(u.astype(prec), device='cuda').contiguous()
+        rb = torch.as_tensor(rbar.astype(prec), device='cuda').contiguous()
+        got = ctx.residual_vjp(ud, rb).cpu().numpy().astype(np.float64)
+        errs = [rel_l2(got[s], ref[s]) for s in (slice(0, 1), slice(1, 4), slice(4, 5))]
+        print('%s %s residual VJP rel-L2 rho/rhoU/rhoE %s' % (name, np.dtype(prec).name, ' '.join('%.2e' % e for e in errs)))
+        assert max(errs) < tol
+
+
+@pytest.mark.parametrize('name', sorted(CASES))
+def test_adjoint_step_matches_autograd(name):
+    case, u = CASES[name]()
+    rng = np.random.default_rng(7)
+    ua = scaled_rbar(case, u, 2) * 1e3
+    src = rng.normal(size=u.shape) * np.abs(u).mean(1, keepdims=True) * 10
+    dt = 1e-6 if name.startswith('mixed') else 2e-5
+    obja = 1.0 if case.objective else 0.0
+    ref_u, ref_s = oracle_step_vjp(case, u, dt, ua, obja, source=src)
+    for prec, tol in ((np.float64, 1e-11), (np.float32, 5e-5)):
+        ctx = case.context(prec)
+        ud = torch.as_tensor(u.astype(prec), device='cuda').contiguous()
+        uad = torch.as_tensor(ua.astype(prec), device='cuda').contiguous()
+        sd = torch.as_tensor(src.astype(prec), device='cuda').contiguous()
+        sg = torch.zeros_like(ud)
+        ctx.adjoint_step(ud, dt, uad, obja=obja, source=sd, source_grad=sg)
+        got = uad.cpu().numpy().astype(np.float64)
+        errs = [rel_l2(got[s], ref_u[s]) for s in (slice(0, 1), slice(1, 4), slice(4, 5))]
+        es = rel_l2(sg.cpu().numpy(), ref_s)
+        print('%s %s adjoint step rel-L2 rho/rhoU/rhoE %s source %.2e' % (name, np.dtype(prec).name,
+              ' '.join('%.2e' % e for e in errs), es))
+        assert max(errs) < tol and es < tol
+        # the primal state is untouched
+        assert np.array_equal(ud.cpu().numpy(), u.astype(prec))
+
+
+def test_multistep_gradient_vs_central_fd_fp64():
+    """d/d eps of sum_n J(u_n^(1)) over 6 steps along a random initial-velocity
+    perturbation: adjoint (6 reverse steps) vs central FD of the HIP primal,
+    FP64, relative difference <= 1e-6."""
+    case, params = cf.tgv(6, seed=2)
+    u0 = cf.tgv_state(case, params)
+    ctx = case.context(np.float64)
+    nsteps, dt = 6, 4e-4
+    rng = np.random.default_rng(3)
+    dU = rng.normal(size=(3, ctx.nIC))
+    rho0 = u0[0]
+    v = np.zeros_like(u0)
+    v[1:4] = rho0 * dU                             # perturb rhoU by rho*dU
+    v[4] = (u0[1:4] * dU).sum(0)                   # first-order change of rhoE at fixed rho, p
+
+    def J(u):
+        ud = torch.as_tensor(u, device='cuda').contiguous()
+        tot = 0.0
+        obj = torch.zeros(1, dtype=torch.float64, device='cuda')
+        for _ in range(nsteps):
+            ctx.primal_step(ud, dt, obj_out=obj)
+            tot += obj.item()
+        return tot
+
+    # forward storing states, then reverse
+    ud = torch.as_tensor(u0, device='cuda').contiguous()
+    states = []
+    for _ in range(nsteps):
+        states.append(ud.clone())
+        ctx.primal_step(ud, dt)
+    ua = torch.zeros_like(ud)
+    for k in reversed(range(nsteps)):
+        ctx.adjoint_step(states[k], dt, ua, obja=1.0)
+    adj = float((ua.cpu().numpy() * v).sum())
+    eps = 1e-6
+    fd = (J(u0 + eps * v) - J(u0 - eps * v)) / (2 * eps)
+    print('multistep: adjoint %.12e fd %.12e rel %.2e' % (adj, fd, abs(adj - fd) / abs(fd)))
+    assert abs(adj - fd) <= 1e-6 * abs(fd)
+
+
+def test_adjoint_deterministic_bitwise():
+    case, params = cf.tgv(6, seed=4)
+    u = cf.tgv_state(case, params)
+    ctx = case.context(np.float32)
+    ud = torch.as_tensor(u.astype(np.float32), device='cuda').contiguous()
+    outs = []
+    for _ in range(3):
+        ua = torch.ones_like(ud) * 1e-3
+        ctx.adjoint_step(ud, 2e-5, ua, obja=1.0)
+        outs.append(ua.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
