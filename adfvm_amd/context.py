@@ -229,6 +229,22 @@ class Context(object):
         self._chk(self.lib.adfvm_adjoint_energy(self.handle, _dptr(ua), _dptr(u), _dptr(out), _stream(stream)))
         return out
 
+    def profile_start(self):
+        self._chk(self.lib.adfvm_profile_reset(self.handle))
+        self._chk(self.lib.adfvm_profile(self.handle, 1, _stream(None)))
+
+    def profile_stop(self):
+        """Returns {kernel name: (total ms, launches)} recorded since
+        profile_start (HIP events on the launch stream)."""
+        self._chk(self.lib.adfvm_profile(self.handle, 0, _stream(None)))
+        out = {}
+        for i in range(self.lib.adfvm_profile_count(self.handle)):
+            buf = C.create_string_buffer(64)
+            ms, n = C.c_double(), C.c_int64()
+            self._chk(self.lib.adfvm_profile_entry(self.handle, i, buf, 64, C.byref(ms), C.byref(n)))
+            out[buf.value.decode()] = (ms.value, n.value)
+        return out
+
     def pack(self, rho, rhoU, rhoE, out=None, stream=None):
         out = out if out is not None else self.state()
         self._chk(self.lib.adfvm_pack_state(self.handle, _dptr(rho), _dptr(rhoU), _dptr(rhoE), _dptr(out),

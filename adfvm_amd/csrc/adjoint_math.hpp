@@ -21,25 +21,26 @@ template <typename T> ADFVM_HD T viscosity_dT(const Phys<T>& ph, T Tf) {
     return ph.muA * (T(1.5) * s * d - Tf * s) / (d * d);
 }
 
-// primitive(q) -> w ; qb += (dw/dq)^T wb
+// primitive(q) -> w ; qb += (dw/dq)^T wb.
+// The chain rule through e = E - |U|^2/2 gives for rho the terms
+//   pb*(g-1)*e - eb*E/rho - sum_i Ub_i U_i/rho,  eb = Tb/Cv + pb*(g-1)*rho,
+// whose p parts cancel down to pb*(g-1)|U|^2/2 (e ~ 1e3 |U|^2 at low Mach);
+// they are summed in that cancelled form, which FP32 evaluates accurately:
+//   rhob = (g-1)/2 |U|^2 pb + (|U|^2/2 - e) Tb/(Cv rho) - (U.wbU)/rho
 template <typename T> ADFVM_HD void primitive_vjp(const Phys<T>& ph, const T* q, const T* wb, T* qb) {
     T rho = q[0];
     T U[3] = {q[1] / rho, q[2] / rho, q[3] / rho};
     T E = q[4] / rho;
-    T e = E - T(0.5) * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    T U2 = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+    T e = E - T(0.5) * U2;
     T gm1 = ph.gamma - T(1);
-    T eb = wb[3] * ph.invCv + wb[4] * gm1 * rho;
-    T rhob = wb[4] * gm1 * e;
-    T Eb = eb;
-    T Ub[3];
-    for (int i = 0; i < 3; i++) Ub[i] = wb[i] - eb * U[i];
-    qb[4] += Eb / rho;
-    rhob += -Eb * E / rho;
-    for (int i = 0; i < 3; i++) {
-        qb[1 + i] += Ub[i] / rho;
-        rhob += -Ub[i] * U[i] / rho;
-    }
-    qb[0] += rhob;
+    T Tb = wb[3] * ph.invCv;                 // adjoint of e through T
+    T pb = wb[4];
+    // eb = Tb + pb*gm1*rho ; qb4 = eb/rho
+    qb[4] += Tb / rho + pb * gm1;
+    for (int i = 0; i < 3; i++) qb[1 + i] += wb[i] / rho - (Tb / rho + pb * gm1) * U[i];
+    T UwU = U[0] * wb[0] + U[1] * wb[1] + U[2] * wb[2];
+    qb[0] += T(0.5) * gm1 * U2 * pb + (T(0.5) * U2 - e) * Tb / rho - UwU / rho;
 }
 
 // conservative(w) -> q ; wb += (dq/dw)^T qb
@@ -70,17 +71,14 @@ ADFVM_HD void reconstruct_vjp(const T* wC, const T* gC, const T* wD, T lw, const
             T v = wC[i] + (wD[i] - wC[i]) * lw + (qw[0] * g[0] + qw[1] * g[1] + qw[2] * g[2]);
             T lo = wC[i] < wD[i] ? wC[i] : wD[i];
             T hi = wC[i] < wD[i] ? wD[i] : wC[i];
-            // out = min(max(v, lo), hi)
-            if (v < lo) {          // lo = min(wC, wD)
-                if (wC[i] < wD[i]) wCb[i] += b;
-                else if (wD[i] < wC[i]) wDb[i] += b;
-                else { wCb[i] += T(0.5) * b; wDb[i] += T(0.5) * b; }
+            // out = v < lo ? lo : (v > hi ? hi : v), lo/hi selected as in
+            // reconstruct(); the adjoint follows the selected value
+            if (v < lo) {
+                if (wC[i] < wD[i]) wCb[i] += b; else wDb[i] += b;
                 continue;
             }
             if (v > hi) {
-                if (wC[i] > wD[i]) wCb[i] += b;
-                else if (wD[i] > wC[i]) wDb[i] += b;
-                else { wCb[i] += T(0.5) * b; wDb[i] += T(0.5) * b; }
+                if (wC[i] < wD[i]) wDb[i] += b; else wCb[i] += b;
                 continue;
             }
         }

@@ -487,9 +487,40 @@ struct adfvm_ctx {
     void* adjGhost = nullptr;  // [5][nBF]
     double* partial = nullptr; int nPartial = 0;
     double* scal = nullptr;    // small device scalars
+    // per-kernel event timing (adfvm_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> evPool;
+    struct Pending { const char* name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<std::string> profNames;
+    std::vector<double> profMs;
+    std::vector<long long> profCount;
 };
 
 namespace {
+
+// Records a start/stop event pair around the kernel launched in its scope
+// when the context's profiling is on (on the launch stream, so the timing is
+// that kernel's execution on that stream).
+struct KTimer {
+    adfvm_ctx* c; const char* name; hipStream_t s; hipEvent_t a = nullptr, b = nullptr;
+    static hipEvent_t take(adfvm_ctx* c) {
+        if (!c->evPool.empty()) { hipEvent_t e = c->evPool.back(); c->evPool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    KTimer(adfvm_ctx* c_, const char* n, hipStream_t s_) : c(c_), name(n), s(s_) {
+        if (!c->prof) return;
+        a = take(c); b = take(c);
+        if (a) (void)hipEventRecord(a, s);
+    }
+    ~KTimer() {
+        if (!c->prof || !a || !b) return;
+        (void)hipEventRecord(b, s);
+        c->pending.push_back({name, a, b});
+    }
+};
 
 int set_err(adfvm_ctx* ctx, int code, const std::string& m) {
     if (ctx) ctx->err = m;
@@ -594,14 +625,17 @@ template <typename T> T* P(void* p) { return (T*)p; }
 // ghost refresh for state u (no-op for meshes with only cyclic patches)
 template <typename T> void launch_ghosts(adfvm_ctx* c, const T* u, hipStream_t s) {
     if (!c->hasGhostKernel) return;
+    KTimer kt(c, "ghost_kernel", s);
     ghost_kernel<T><<<grid_for(c->nBF), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost));
 }
 
 template <typename T> void launch_grad(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
+    KTimer kt(c, "grad_kernel", s);
     grad_kernel<T><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), grad);
 }
 
 template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* out, hipStream_t s) {
+    KTimer kt(c, "objective", s);
     if (c->objective == ADFVM_OBJ_KINETIC_ENERGY) {
         int g = grid_for(c->nIC);
         ke_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), u, c->partial);
@@ -645,9 +679,12 @@ int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, do
         const bool wantDtc = (i == objStage) && dtc_out;
         const T* u0 = (i > 0 && st[i].a0 != 0.0) ? (const T*)u : nullptr;
         int g = grid_for(c->nIC);
-        flux_kernel<T, 0><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), ui, grads[i], P<T>(c->ghost), u0,
-                                               (T)st[i].a0, (T)st[i].ai, (T)(-st[i].beta), (T)dt, S, out, nullptr,
-                                               wantDtc ? c->partial : nullptr);
+        {
+            KTimer kt(c, "flux_kernel", s);
+            flux_kernel<T, 0><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), ui, grads[i], P<T>(c->ghost),
+                                                   u0, (T)st[i].a0, (T)st[i].ai, (T)(-st[i].beta), (T)dt, S, out,
+                                                   nullptr, wantDtc ? c->partial : nullptr);
+        }
         if (wantDtc) finalize_kernel<1><<<1, BLOCK, 0, s>>>(c->partial, g, dtc_out);
         ui = out;
     }
@@ -775,6 +812,8 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->partial, c->scal};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : c->evPool) (void)hipEventDestroy(e);
     delete c;
 }
 

@@ -1,0 +1,228 @@
+"""Benchmark: compressible FV step + discrete adjoint on one MI355X.
+
+Workload: 3-D Taylor-Green vortex on a periodic 128^3 hex box of side 2*pi,
+stored as an unstructured face list (seeded random cell permutation, then
+reverse Cuthill-McKee), Mach 0.1, rho0 = 1.2, Re = 1600, Roe flux, SSP-RK3,
+fixed dt.  One "step" is one time step forward plus its reverse (adjoint)
+step: K forward steps keep their start states in HBM, then K adjoint steps
+run in reverse (each re-evaluates its RK stages from the stored state).
+
+Reported value: FP32 primal+adjoint cell-updates/s (nCells * K / time).
+Also reported: the same for FP64, primal-only rates, the FP32-vs-FP64
+relative L2 error per conserved variable after --l2-steps primal steps, the
+roofline of the dominant kernel (HIP events on the library's stream) and the
+CPU oracle timed on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+
+
+def kernel_bytes(name, s, nC, nF, nG):
+    """Algorithmic bytes one launch moves: each array element the kernel
+    needs counted once (s = scalar bytes; slot tables 6 x (int32 face,
+    int32 neighbour, uint8 flags) = 54 B per cell)."""
+    slots = 54 * nC
+    if name == 'grad_kernel':          # u, V, face (A, n, w) -> 15 gradients
+        return nC * (5 + 1 + 15) * s + nF * 5 * s + slots
+    if name == 'flux_kernel':          # u_i, grad, u0 (2 of 3 stages), V, 17 face rows -> u_{i+1}
+        return nC * (5 + 15 + 10. / 3 + 1 + 5) * s + nF * 17 * s + slots
+    if name == 'flux_vjp_kernel':      # u, grad, ubar_{i+1}, V, 17 face rows -> wbar, gbar
+        return nC * (5 + 15 + 5 + 1 + 20) * s + nF * 17 * s + slots
+    if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, face (A,n,w) -> ubar, acc
+        return nC * (5 + 5 + 15 + 5 + 10. / 3 + 1 + 5 + 10. / 3) * s + nF * 5 * s + slots
+    return None
+
+
+def step_bytes_model(s, nC, nF, nG):
+    """Primal and primal+adjoint bytes per cell-update, the survey's model:
+    B_P = s[nC(5+5+10/3+15+15+1) + 40 nG + 17 nF] + 8 nF per stage,
+    B_A = s[nC(5+15+5+5+20/3+15+15+1) + 40 nG + 17 nF] + 8 nF per stage."""
+    BP = s * (nC * (5 + 5 + 10. / 3 + 15 + 15 + 1) + 40 * nG + 17 * nF) + 8 * nF
+    BA = s * (nC * (5 + 15 + 5 + 5 + 20. / 3 + 15 + 15 + 1) + 40 * nG + 17 * nF) + 8 * nF
+    return 3 * BP / nC, 3 * (BP + BA) / nC
+
+
+def run_precision(case, u0, prec, K, W, dt, torch, profile):
+    ctx = case.context(prec)
+    nC = ctx.nIC
+    u = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
+    store = torch.empty((K, 5, nC), dtype=ctx.tdtype, device='cuda')
+    ua = torch.zeros_like(u)
+
+    def pa_pass(k, prof=False):
+        if prof:
+            ctx.profile_start()
+        for i in range(k):
+            store[i].copy_(u)
+            ctx.primal_step(u, dt)
+        ua.zero_()
+        for i in reversed(range(k)):
+            ctx.adjoint_step(store[i], dt, ua, obja=1.0)
+        return ctx.profile_stop() if prof else None
+
+    pa_pass(min(W, K))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prof = pa_pass(K, prof=profile)
+    torch.cuda.synchronize()
+    t_pa = time.perf_counter() - t0
+    ok = bool(torch.isfinite(u).all()) and bool(torch.isfinite(ua).all())
+    # primal only
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        ctx.primal_step(u, dt)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t0
+    del ctx
+    return dict(t_pa=t_pa, t_p=t_p, prof=prof, finite=ok)
+
+
+def l2_error(case, u0, steps, dt, torch):
+    out = {}
+    for prec in (np.float32, np.float64):
+        ctx = case.context(prec)
+        u = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
+        for _ in range(steps):
+            ctx.primal_step(u, dt)
+        out[prec] = u.double().cpu().numpy()
+        del ctx
+    a, b = out[np.float32], out[np.float64]
+    V = case.mesh.volumes[:, 0]
+    err = {}
+    for name, sl in (('rho', slice(0, 1)), ('rhoU', slice(1, 4)), ('rhoE', slice(4, 5))):
+        d = ((a[sl] - b[sl]) ** 2 * V).sum()
+        n = (b[sl] ** 2 * V).sum()
+        err[name] = float(np.sqrt(d / n))
+    return err
+
+
+def cpu_baseline(n=32, steps=8, dt=2e-4):
+    """The oracle (torch float64 on the host, autograd reverse) on a bounded
+    sample: TGV n^3, `steps` steps forward + reverse."""
+    import torch
+    from tests import casefactory as cf
+    case, params = cf.tgv(n, seed=0)
+    u0 = cf.tgv_state(case, params)
+    oc = case.oracle()
+    x = torch.tensor(u0, dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    r, rU, rE = x[0:1].T, x[1:4].T, x[4:5].T
+    J = 0
+    for _ in range(steps):
+        r, rU, rE, dtc, obj = oc.step(r, rU, rE, dt)
+        J = J + obj
+    (g,) = torch.autograd.grad(J, x)
+    t = time.perf_counter() - t0
+    nC = case.mesh.nInternalCells
+    return dict(value=nC * steps / t, unit='cell-updates/s primal+adjoint', cores=torch.get_num_threads(),
+                kind='port', sample='oracle (torch float64, autograd reverse) TGV %d^3, %d steps forward+reverse, '
+                                    '%.1f s' % (n, steps, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--n', type=int, default=128)
+    ap.add_argument('--l2-steps', type=int, default=100)
+    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)' % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from tests import casefactory as cf
+    assert torch.cuda.is_available(), 'bench.py needs the GPU'
+    torch.cuda.set_device(0)
+
+    t0 = time.time()
+    case, params = cf.tgv(args.n, seed=0)
+    m = case.mesh
+    nC, nF = m.nInternalCells, m.nFaces
+    nG = m.nFaces - m.nInternalFaces
+    u0 = cf.tgv_state(case, params)
+    dx = 2 * np.pi / args.n
+    c0 = np.sqrt(cf.GAMMA * cf.R * params['T0'])
+    dt = 0.5 * dx / (params['U0'] + c0)       # fixed dt at CFL 0.5 (convective + acoustic)
+    print('[bench] mesh %d^3: %d cells, %d faces built in %.1fs; dt=%.3e' % (args.n, nC, nF, time.time() - t0, dt),
+          file=sys.stderr, flush=True)
+
+    r32 = run_precision(case, u0, np.float32, args.steps, args.warmup, dt, torch, profile=True)
+    print('[bench] fp32 P+A %.3fs primal %.3fs' % (r32['t_pa'], r32['t_p']), file=sys.stderr, flush=True)
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_precision(case, u0, np.float64, args.steps, args.warmup, dt, torch, profile=True)
+        print('[bench] fp64 P+A %.3fs primal %.3fs' % (r64['t_pa'], r64['t_p']), file=sys.stderr, flush=True)
+    l2 = l2_error(case, u0, args.l2_steps, dt, torch) if not args.no_fp64 else None
+    print('[bench] L2 %s' % l2, file=sys.stderr, flush=True)
+
+    K = args.steps
+    value = nC * K / r32['t_pa']
+    # dominant kernel roofline (FP32)
+    prof = r32['prof']
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms_tot, launches = prof[dom]
+    avg_s = ms_tot / launches * 1e-3
+    kb = kernel_bytes(dom, 4, nC, nF, nG)
+    achieved = kb / avg_s / 1e9 if kb else None
+    traffic = None
+    tf = os.path.join(ROOT, 'profiles', 'traffic.json')
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get(dom)
+    bpc_p32, bpc_pa32 = step_bytes_model(4, nC, nF, nG)
+    bpc_p64, bpc_pa64 = step_bytes_model(8, nC, nF, nG)
+    out = {
+        'metric': 'cell-updates/s primal+adjoint, FP32 vs FP64, % HBM roofline, L2 err vs FP64',
+        'value': value,
+        'unit': 'cell-updates/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': args.warmup,
+        'ms_per_step': r32['t_pa'] / K * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (Taylor-Green initial field)',
+        'config': {'workload': 'TGV %d^3 periodic hex, random permutation + RCM, primal+adjoint, SSP-RK3, Roe, '
+                               'fixed dt, start states of all K steps kept in HBM' % args.n,
+                   'n_cells': nC, 'n_faces': nF, 'dt': dt, 'objective': 'kinetic energy on the stage-1 state'},
+        'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+                     'frac': (achieved / (HBM_PEAK / 1e9)) if achieved else None, 'traffic': traffic,
+                     'algorithmic_bytes_per_launch': kb, 'avg_launch_ms': avg_s * 1e3},
+        'hbm_frac_step_model': value * bpc_pa32 / HBM_PEAK,
+        'bytes_per_cell_update_model': {'fp32_primal': bpc_p32, 'fp32_pa': bpc_pa32, 'fp64_pa': bpc_pa64},
+        'primal_cell_updates_per_s_fp32': nC * K / r32['t_p'],
+        'kernel_ms_fp32': {k: round(v[0] / K, 4) for k, v in prof.items()},
+        'finite': r32['finite'],
+    }
+    if r64 is not None:
+        v64 = nC * K / r64['t_pa']
+        out.update({'value_fp64': v64, 'fp32_over_fp64': value / v64,
+                    'hbm_frac_step_model_fp64': v64 * bpc_pa64 / HBM_PEAK,
+                    'primal_cell_updates_per_s_fp64': nC * K / r64['t_p'],
+                    'kernel_ms_fp64': {k: round(v[0] / K, 4) for k, v in r64['prof'].items()},
+                    'l2_rel_err_fp32_vs_fp64': l2, 'l2_steps': args.l2_steps})
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

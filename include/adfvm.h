@@ -242,6 +242,17 @@ int adfvm_dot(adfvm_ctx* ctx, const void* a, const void* b, int64_t n, double* o
 /* adjoint energy sum_c w^T A0 w V_c with w = ua/V (cmesh.cpp:342-425). */
 int adfvm_adjoint_energy(adfvm_ctx* ctx, const void* ua, const void* u, double* out, void* stream);
 
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * each hot-path kernel (replaces the reference's wall-clock prints,
+ * adFVM/solver.py:309-346, at kernel granularity).  enable = 1 starts
+ * recording; enable = 0 synchronises the stream and folds the recorded
+ * launches into per-kernel totals, read with adfvm_profile_entry. */
+int adfvm_profile(adfvm_ctx* ctx, int32_t enable, void* stream);
+int adfvm_profile_count(const adfvm_ctx* ctx);
+int adfvm_profile_entry(const adfvm_ctx* ctx, int32_t i, char* name, int32_t len, double* total_ms,
+                        int64_t* launches);
+int adfvm_profile_reset(adfvm_ctx* ctx);
+
 /* Layout helpers: reference layout (rho nIC, rhoU nIC x 3 row-major,
  * rhoE nIC; device, context precision) <-> state u[5][nIC]. */
 int adfvm_pack_state(adfvm_ctx* ctx, const void* rho, const void* rhoU, const void* rhoE,

@@ -228,8 +228,11 @@ class Case(object):
         gC = grad[gsrc[p]]                                       # (nf,k,3)
         out = phiC + (phiD - phiC) * lw + (gC * qw[:, None, :]).sum(-1)
         if self.limiter:
-            # optional clamp between the two cell values (not in the reference)
-            out = torch.minimum(torch.maximum(out, torch.minimum(phiC, phiD)), torch.maximum(phiC, phiD))
+            # optional clamp between the two cell values (not in the reference);
+            # torch.where so that ties pass the adjoint to one selected value
+            lo = torch.where(phiC < phiD, phiC, phiD)
+            hi = torch.where(phiC < phiD, phiD, phiC)
+            out = torch.where(out < lo, lo, torch.where(out > hi, hi, out))
         return out
 
     # --------------------------------------------------- Riemann solvers --

@@ -104,7 +104,7 @@ def test_interp_second_order_like_reference():
     """tests/test_interp.py:19-48: secondOrder of 2x + x^3 y^2 with Gauss
     gradients; max-relative error 1e-5 against the exact face values."""
     from adfvm_amd import meshgen
-    case = cf.CaseDef(meshgen.periodic_box((40, 40, 4), (1., 1., 0.1)), {}, 0.)
+    case = cf.CaseDef(meshgen.periodic_box((100, 100, 10), (1., 1., 0.1)), {}, 0.)   # the convection case's grid
     m = case.mesh
     X, Y = m.cellCentres[:, 0], m.cellCentres[:, 1]
     Uc = 2 * X + X ** 3 * Y ** 2
