@@ -50,6 +50,8 @@ template <typename T> struct DMesh {
     const unsigned char* sFlags;  // [6][nIC]
     const int* sPair;       // [6][nIC]: paired face of a coupled slot, else -1
     const int* owner;       // [nF]
+    const int* fOther;      // [nF]: neighbour, paired internal cell (cyclic) or -1-b (ghost)
+    const unsigned char* fKind;  // [nF]: FACE_INTERIOR / FACE_CHARACTERISTIC / FACE_BOUNDARY
     const int* bPatch;      // [nBF]
     const DPatch* patches;
     const T* bcval;
@@ -68,18 +70,20 @@ template <typename T> __device__ __forceinline__ void load_geo(const DMesh<T>& m
     g.qw1[0] = G[G_Q1X * s + f]; g.qw1[1] = G[G_Q1Y * s + f]; g.qw1[2] = G[G_Q1Z * s + f];
 }
 
-// primitive values of a value source: internal cell (from the state) or ghost
+// primitive values of a value source: internal cell (from the state) or ghost.
+// Branch-free (one strided load from either source, then a per-element
+// select) so that w stays in registers: writing the array in two branches
+// made the compiler place it in scratch memory.
 template <typename T>
 __device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict__ u, long nIC,
                                           const T* __restrict__ ghost, long nBF, int src, T* w) {
-    if (src >= 0) {
-        T q[5];
-        for (int i = 0; i < 5; i++) q[i] = u[i * nIC + src];
-        primitive(ph, q, w);
-    } else {
-        int b = -1 - src;
-        for (int i = 0; i < 5; i++) w[i] = ghost[i * nBF + b];
-    }
+    const bool internal = src >= 0;
+    const T* base = internal ? u + src : ghost + (-1 - src);
+    const long stride = internal ? nIC : nBF;
+    T q[5], p[5];
+    for (int i = 0; i < 5; i++) q[i] = base[i * stride];
+    primitive(ph, q, p);
+    for (int i = 0; i < 5; i++) w[i] = internal ? p[i] : q[i];
 }
 
 template <typename T>
@@ -209,51 +213,68 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(const double* __restric
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-// Residual and stage update, one cell per thread:
-//   L_c = sum_slots sign*F_f*(A_f/V_c)   (op.div, op.py:12-20)
-//   lam_c = sum_slots lam_f*(A_f/V_c)    (op.absDiv, op.py:22-29)
+// Face fluxes, one face per thread (faces are in owner order, so the owner
+// reads of neighbouring threads coincide and the geometry rows stream):
+//   fbuf[i][f] = F_i(f)*A_f (i < 5),  fbuf[5][f] = lam_f*A_f
+// Internal and coupled faces: RCF.flux (density.py:253-305) between the
+// owner and the neighbour (for a cyclic face, the paired internal cell);
+// characteristic / other boundary faces: the ghost as the right state.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) face_flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+                                                          const T* __restrict__ grad, const T* __restrict__ ghost,
+                                                          T* __restrict__ fbuf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
+    const int P = m.owner[f];
+    const int o = m.fOther[f];
+    const int kind = m.fKind[f];
+    FaceGeo<T> fg;
+    load_geo(m, f, fg);
+    T wP[5], gP[15], wN[5], gN[15], F[5], lam;
+    load_prim(ph, u, nIC, ghost, nBF, P, wP);
+    load_grad(grad, nIC, P, gP);
+    load_prim(ph, u, nIC, ghost, nBF, o, wN);
+    if (kind == FACE_INTERIOR) {
+        load_grad(grad, nIC, o, gN);
+        face_flux(ph, FACE_INTERIOR, wP, gP, wN, gN, fg, F, &lam);
+    } else {
+        face_flux(ph, kind, wP, gP, wN, gP, fg, F, &lam);
+    }
+    for (int i = 0; i < 5; i++) fbuf[i * nF + f] = F[i] * fg.A;
+    fbuf[5 * nF + f] = lam * fg.A;
+}
+
+// Residual and stage update, one cell per thread, gathering the six face
+// fluxes in cellFaces slot order:
+//   L_c = (1/V_c) sum_slots sign*(F A)_f     (op.div, op.py:12-20)
+//   lam_c = (1/V_c) sum_slots (lam A)_f      (op.absDiv, op.py:22-29)
 // MODE 0: uout = a0*u0 + ai*ui + (-beta*(L-S))*dt   (timestep.update, timestep.py:35-45)
 // MODE 1: uout = L (the residual)
 template <typename T, int MODE>
-__global__ void __launch_bounds__(BLOCK) flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ ui,
-                                                     const T* __restrict__ grad, const T* __restrict__ ghost,
-                                                     const T* __restrict__ u0, T a0, T ai, T nbeta, T dt,
-                                                     const T* __restrict__ S, T* __restrict__ uout,
-                                                     T* __restrict__ dtcCell, double* __restrict__ dtcPartial) {
+__global__ void __launch_bounds__(BLOCK) stage_kernel(DMesh<T> m, const T* __restrict__ fbuf,
+                                                      const T* __restrict__ ui, const T* __restrict__ u0, T a0,
+                                                      T ai, T nbeta, T dt, const T* __restrict__ S,
+                                                      T* __restrict__ uout, T* __restrict__ dtcCell,
+                                                      double* __restrict__ dtcPartial) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const long nIC = m.nIC, nBF = m.nBF;
+    const long nIC = m.nIC, nF = m.nF;
     double dtcv = 0.0;
     if (c < m.nIC) {
-        T wc[5], gc[15];
-        load_prim(ph, ui, nIC, ghost, nBF, c, wc);
-        load_grad(grad, nIC, c, gc);
-        const T invV = T(1);
         T L[5] = {0, 0, 0, 0, 0};
         T lamSum = 0;
-        const T V = m.vol[c];
-        (void)invV;
         for (int j = 0; j < 6; j++) {
             const int f = m.sFace[j * nIC + c];
-            const int o = m.sOther[j * nIC + c];
-            const int fl = m.sFlags[j * nIC + c];
-            const bool own = fl & 1;
-            const int kind = fl >> 1;
-            FaceGeo<T> fg;
-            load_geo(m, f, fg);
-            T wo[5], go[15], F[5], lam;
-            load_prim(ph, ui, nIC, ghost, nBF, o, wo);
-            if (kind == SLOT_INTERNAL || kind == SLOT_COUPLED) {
-                load_grad(grad, nIC, o, go);
-                if (own) face_flux(ph, FACE_INTERIOR, wc, gc, wo, go, fg, F, &lam);
-                else face_flux(ph, FACE_INTERIOR, wo, go, wc, gc, fg, F, &lam);
-            } else {
-                face_flux(ph, kind == SLOT_CHARACTERISTIC ? FACE_CHARACTERISTIC : FACE_BOUNDARY, wc, gc, wo, gc, fg, F, &lam);
+            const bool own = m.sFlags[j * nIC + c] & 1;
+            for (int i = 0; i < 5; i++) {
+                const T v = fbuf[i * nF + f];
+                L[i] += own ? v : -v;
             }
-            T wgt = fg.A / V;
-            T sw = own ? wgt : -wgt;
-            for (int i = 0; i < 5; i++) L[i] += F[i] * sw;
-            lamSum += lam * wgt;
+            lamSum += fbuf[5 * nF + f];
         }
+        const T V = m.vol[c];
+        for (int i = 0; i < 5; i++) L[i] = L[i] / V;
+        lamSum = lamSum / V;
         if (MODE == 1) {
             for (int i = 0; i < 5; i++) uout[i * nIC + c] = L[i];
         } else {
@@ -487,6 +508,9 @@ struct adfvm_ctx {
     void* adjGhost = nullptr;  // [5][nBF]
     double* partial = nullptr; int nPartial = 0;
     double* scal = nullptr;    // small device scalars
+    int* fOther = nullptr; unsigned char* fKind = nullptr;
+    void* fbuf = nullptr;      // [6][nF] face fluxes * area
+    void* frec = nullptr;      // [40][nF] face adjoint records (owner side 20, neighbour side 20)
     // per-kernel event timing (adfvm_profile)
     bool prof = false;
     std::vector<hipEvent_t> evPool;
@@ -550,6 +574,7 @@ template <typename T> DMesh<T> dmesh(const adfvm_ctx* c) {
     m.geo = (const T*)c->geo; m.vol = (const T*)c->vol;
     m.sFace = c->sFace; m.sOther = c->sOther; m.sFlags = c->sFlags; m.sPair = c->sPair;
     m.owner = c->owner; m.bPatch = c->bPatch; m.patches = c->patches; m.bcval = (const T*)c->bcval;
+    m.fOther = c->fOther; m.fKind = c->fKind;
     return m;
 }
 
@@ -617,6 +642,8 @@ template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* 
     HIPCHK(ctx, hipMalloc(&ctx->adjG, gr));
     HIPCHK(ctx, hipMalloc(&ctx->adjW, st));
     HIPCHK(ctx, hipMalloc(&ctx->adjGhost, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->fbuf, 6 * nF * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->frec, 40 * nF * sizeof(T)));
     return ADFVM_OK;
 }
 
@@ -632,6 +659,23 @@ template <typename T> void launch_ghosts(adfvm_ctx* c, const T* u, hipStream_t s
 template <typename T> void launch_grad(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
     KTimer kt(c, "grad_kernel", s);
     grad_kernel<T><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), grad);
+}
+
+template <typename T> void launch_face_flux(adfvm_ctx* c, const T* u, const T* grad, hipStream_t s) {
+    KTimer kt(c, "face_flux_kernel", s);
+    face_flux_kernel<T><<<grid_for(c->nF), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, grad, P<T>(c->ghost),
+                                                           P<T>(c->fbuf));
+}
+
+struct Stage { double a0, ai, beta; };   // u_{i+1} = a0*u0 + ai*u_i - beta*(L-S)*dt
+
+template <typename T, int MODE>
+void launch_stage(adfvm_ctx* c, const T* ui, const T* u0, const Stage& st, double dt, const T* S, T* out,
+                  T* dtcCell, double* dtcPartial, hipStream_t s) {
+    KTimer kt(c, MODE == 0 ? "stage_kernel" : "residual_kernel", s);
+    stage_kernel<T, MODE><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.a0,
+                                                              (T)st.ai, (T)(-st.beta), (T)dt, S, out, dtcCell,
+                                                              dtcPartial);
 }
 
 template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* out, hipStream_t s) {
@@ -650,7 +694,6 @@ template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* ou
     }
 }
 
-struct Stage { double a0, ai, beta; };   // u_{i+1} = a0*u0 + ai*u_i - beta*(L-S)*dt
 void stages(int integrator, std::vector<Stage>& st) {
     st.clear();
     if (integrator == ADFVM_RK_EULER) { st.push_back({0, 1, 1}); return; }
@@ -679,12 +722,8 @@ int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, do
         const bool wantDtc = (i == objStage) && dtc_out;
         const T* u0 = (i > 0 && st[i].a0 != 0.0) ? (const T*)u : nullptr;
         int g = grid_for(c->nIC);
-        {
-            KTimer kt(c, "flux_kernel", s);
-            flux_kernel<T, 0><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), ui, grads[i], P<T>(c->ghost),
-                                                   u0, (T)st[i].a0, (T)st[i].ai, (T)(-st[i].beta), (T)dt, S, out,
-                                                   nullptr, wantDtc ? c->partial : nullptr);
-        }
+        launch_face_flux<T>(c, ui, grads[i], s);
+        launch_stage<T, 0>(c, ui, u0, st[i], dt, S, out, nullptr, wantDtc ? c->partial : nullptr, s);
         if (wantDtc) finalize_kernel<1><<<1, BLOCK, 0, s>>>(c->partial, g, dtc_out);
         ui = out;
     }
@@ -785,8 +824,25 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
             cellNb[j * nIC + cc] = d->cellNeighbours[6 * cc + j];
         }
     }
+    // face tables: the neighbour side's value source and the flux kind
+    std::vector<int> fOther(nF);
+    std::vector<unsigned char> fKind(nF);
+    for (long f = 0; f < nF; f++) {
+        if (f < nIF) { fOther[f] = d->neighbour[f]; fKind[f] = FACE_INTERIOR; continue; }
+        const long b = f - nIF;
+        const adfvm_patch_desc& Pd = d->patches[bPatch[b]];
+        if (Pd.fluxKind == ADFVM_FLUX_COUPLED) {
+            fOther[f] = d->owner[d->patches[Pd.neighbourPatch].startFace + (f - Pd.startFace)];
+            fKind[f] = FACE_INTERIOR;
+        } else {
+            fOther[f] = (int)(-1 - b);
+            fKind[f] = Pd.fluxKind == ADFVM_FLUX_CHARACTERISTIC ? FACE_CHARACTERISTIC : FACE_BOUNDARY;
+        }
+    }
     int rc;
     std::vector<int> owner(d->owner, d->owner + nF), nbFull(d->neighbour, d->neighbour + nF);
+    if ((rc = upload(c, (void**)&c->fOther, fOther)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->fKind, fKind)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->sFace, sFace)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->sOther, sOther)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->sFlags, sFlags)) != ADFVM_OK) return rc;
@@ -809,7 +865,7 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
     void* ptrs[] = {c->geo, c->vol, c->bcval, c->sFace, c->sOther, c->sFlags, c->sPair, c->owner, c->bPatch,
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->u3, c->grad0, c->grad1,
                     c->grad2, c->adj0, c->adj1, c->adjAcc, c->adjRbar, c->adjG, c->adjW, c->adjGhost,
-                    c->partial, c->scal};
+                    c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -840,9 +896,8 @@ template <typename T> int gradients_t(adfvm_ctx* c, const T* u, T* grad, hipStre
 template <typename T> int residual_t(adfvm_ctx* c, const T* u, T* R, T* dtcCell, hipStream_t s) {
     launch_ghosts<T>(c, u, s);
     launch_grad<T>(c, u, P<T>(c->grad0), s);
-    flux_kernel<T, 1><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->grad0),
-                                                          P<T>(c->ghost), nullptr, T(0), T(0), T(0), T(0),
-                                                          nullptr, R, dtcCell, nullptr);
+    launch_face_flux<T>(c, u, P<T>(c->grad0), s);
+    launch_stage<T, 1>(c, u, nullptr, Stage{0, 0, 0}, 0.0, nullptr, R, dtcCell, nullptr, s);
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
 }
