@@ -86,9 +86,51 @@ __device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict
     for (int i = 0; i < 5; i++) w[i] = internal ? p[i] : q[i];
 }
 
+// Internal work buffers are arrays of padded records so that a gathered
+// record is a few 16-byte vector loads instead of one cache line per field:
+//   gradients        [nIC][GREC]   15 used
+//   face fluxes      [nF][FREC]    6 used (F*A, lam*A)
+//   face adjoints    [nF][2*AREC]  per side: gradient adjoint (15), primitive adjoint (5)
+//   cell adjoints    [nIC][AREC]   same layout as one side of a face record
+constexpr int GREC = 16, FREC = 8, AREC = 20;
+
+template <typename T> struct Vec16 { using type = typename std::conditional<sizeof(T) == 4, float4, double2>::type; };
+
+// N contiguous values from a 16-byte aligned address (N a multiple of 16/sizeof(T))
+template <int N, typename T> __device__ __forceinline__ void vload(const T* __restrict__ p, T* out) {
+    using V = typename Vec16<T>::type;
+    constexpr int W = 16 / sizeof(T);
+    static_assert(N % W == 0, "vector width");
+    const V* q = reinterpret_cast<const V*>(p);
+#pragma unroll
+    for (int k = 0; k < N / W; k++) {
+        V v = q[k];
+        const T* s = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int i = 0; i < W; i++) out[k * W + i] = s[i];
+    }
+}
+
+template <int N, typename T> __device__ __forceinline__ void vstore(T* __restrict__ p, const T* in) {
+    using V = typename Vec16<T>::type;
+    constexpr int W = 16 / sizeof(T);
+    static_assert(N % W == 0, "vector width");
+    V* q = reinterpret_cast<V*>(p);
+#pragma unroll
+    for (int k = 0; k < N / W; k++) {
+        V v;
+        T* d = reinterpret_cast<T*>(&v);
+#pragma unroll
+        for (int i = 0; i < W; i++) d[i] = in[k * W + i];
+        q[k] = v;
+    }
+}
+
+// the 15 gradient entries of cell c (g has room for GREC values)
 template <typename T>
 __device__ __forceinline__ void load_grad(const T* __restrict__ grad, long nIC, int c, T* g) {
-    for (int i = 0; i < 15; i++) g[i] = grad[i * nIC + c];
+    (void)nIC;
+    vload<GREC>(grad + (long)GREC * c, g);
 }
 
 // ------------------------------------------------------------- kernels ---
@@ -173,7 +215,20 @@ __global__ void __launch_bounds__(BLOCK) grad_kernel(DMesh<T> m, Phys<T> ph, con
         }
     }
     T V = m.vol[c];
-    for (int i = 0; i < 15; i++) grad[i * nIC + c] = g[i] / V;
+    T out[GREC];
+    for (int i = 0; i < 15; i++) out[i] = g[i] / V;
+    out[15] = T(0);
+    vstore<GREC>(grad + (long)GREC * c, out);
+}
+
+// gradient records -> the public [15][nIC] rows
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) grad_rows_kernel(long nIC, const T* __restrict__ grec, T* __restrict__ rows) {
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nIC) return;
+    T g[GREC];
+    vload<GREC>(grec + GREC * c, g);
+    for (int i = 0; i < 15; i++) rows[i * nIC + c] = g[i];
 }
 
 // Block reduction of one double per thread into out[blockIdx.x]; op 0 sum, 1 max.
@@ -231,7 +286,7 @@ __global__ void __launch_bounds__(BLOCK) face_flux_kernel(DMesh<T> m, Phys<T> ph
     const int kind = m.fKind[f];
     FaceGeo<T> fg;
     load_geo(m, f, fg);
-    T wP[5], gP[15], wN[5], gN[15], F[5], lam;
+    T wP[5], gP[GREC], wN[5], gN[GREC], F[5], lam;
     load_prim(ph, u, nIC, ghost, nBF, P, wP);
     load_grad(grad, nIC, P, gP);
     load_prim(ph, u, nIC, ghost, nBF, o, wN);
@@ -241,8 +296,12 @@ __global__ void __launch_bounds__(BLOCK) face_flux_kernel(DMesh<T> m, Phys<T> ph
     } else {
         face_flux(ph, kind, wP, gP, wN, gP, fg, F, &lam);
     }
-    for (int i = 0; i < 5; i++) fbuf[i * nF + f] = F[i] * fg.A;
-    fbuf[5 * nF + f] = lam * fg.A;
+    T rec[FREC];
+    for (int i = 0; i < 5; i++) rec[i] = F[i] * fg.A;
+    rec[5] = lam * fg.A;
+    rec[6] = rec[7] = T(0);
+    vstore<FREC>(fbuf + (long)FREC * f, rec);
+    (void)nF;
 }
 
 // Residual and stage update, one cell per thread, gathering the six face
@@ -266,12 +325,12 @@ __global__ void __launch_bounds__(BLOCK) stage_kernel(DMesh<T> m, const T* __res
         for (int j = 0; j < 6; j++) {
             const int f = m.sFace[j * nIC + c];
             const bool own = m.sFlags[j * nIC + c] & 1;
-            for (int i = 0; i < 5; i++) {
-                const T v = fbuf[i * nF + f];
-                L[i] += own ? v : -v;
-            }
-            lamSum += fbuf[5 * nF + f];
+            T rec[FREC];
+            vload<FREC>(fbuf + (long)FREC * f, rec);
+            for (int i = 0; i < 5; i++) L[i] += own ? rec[i] : -rec[i];
+            lamSum += rec[5];
         }
+        (void)nF;
         const T V = m.vol[c];
         for (int i = 0; i < 5; i++) L[i] = L[i] / V;
         lamSum = lamSum / V;
@@ -500,12 +559,10 @@ struct adfvm_ctx {
     int* cellNbFull = nullptr;   // reference cellNeighbours [6][nIC]
     int* nbFull = nullptr;       // ghost-extended neighbour [nF]
     void* ghost = nullptr;
-    void* u1 = nullptr; void* u2 = nullptr; void* u3 = nullptr;
+    void* u1 = nullptr; void* u2 = nullptr;
     void* grad0 = nullptr; void* grad1 = nullptr; void* grad2 = nullptr;
-    void* adj0 = nullptr; void* adj1 = nullptr; void* adjAcc = nullptr; void* adjRbar = nullptr;
-    void* adjG = nullptr;      // [15][nIC] gradient adjoint
-    void* adjW = nullptr;      // [5][nIC] primitive adjoint
-    void* adjGhost = nullptr;  // [5][nBF]
+    void* adjAcc = nullptr;    // [5][nIC] adjoint of u0 accumulated over the stages
+    void* adjG = nullptr;      // [nIC][AREC] cell adjoint records (gradient 15, primitive 5)
     double* partial = nullptr; int nPartial = 0;
     double* scal = nullptr;    // small device scalars
     int* fOther = nullptr; unsigned char* fKind = nullptr;
@@ -626,24 +683,18 @@ template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* 
     }
     if ((rc = upload(ctx, &ctx->bcval, vals)) != ADFVM_OK) return rc;
     // work buffers
-    size_t st = 5 * nIC * sizeof(T), gr = 15 * nIC * sizeof(T);
+    size_t st = 5 * nIC * sizeof(T), gr = (size_t)GREC * nIC * sizeof(T);
     HIPCHK(ctx, hipMalloc(&ctx->ghost, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
     HIPCHK(ctx, hipMemset(ctx->ghost, 0, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
     HIPCHK(ctx, hipMalloc(&ctx->u1, st));
     HIPCHK(ctx, hipMalloc(&ctx->u2, st));
-    HIPCHK(ctx, hipMalloc(&ctx->u3, st));
     HIPCHK(ctx, hipMalloc(&ctx->grad0, gr));
     HIPCHK(ctx, hipMalloc(&ctx->grad1, gr));
     HIPCHK(ctx, hipMalloc(&ctx->grad2, gr));
-    HIPCHK(ctx, hipMalloc(&ctx->adj0, st));
-    HIPCHK(ctx, hipMalloc(&ctx->adj1, st));
     HIPCHK(ctx, hipMalloc(&ctx->adjAcc, st));
-    HIPCHK(ctx, hipMalloc(&ctx->adjRbar, st));
-    HIPCHK(ctx, hipMalloc(&ctx->adjG, gr));
-    HIPCHK(ctx, hipMalloc(&ctx->adjW, st));
-    HIPCHK(ctx, hipMalloc(&ctx->adjGhost, std::max<size_t>(1, 5 * nBF) * sizeof(T)));
-    HIPCHK(ctx, hipMalloc(&ctx->fbuf, 6 * nF * sizeof(T)));
-    HIPCHK(ctx, hipMalloc(&ctx->frec, 40 * nF * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->adjG, (size_t)AREC * nIC * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->fbuf, (size_t)FREC * nF * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->frec, (size_t)2 * AREC * nF * sizeof(T)));
     return ADFVM_OK;
 }
 
@@ -863,8 +914,8 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
 extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
     if (!c) return;
     void* ptrs[] = {c->geo, c->vol, c->bcval, c->sFace, c->sOther, c->sFlags, c->sPair, c->owner, c->bPatch,
-                    c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->u3, c->grad0, c->grad1,
-                    c->grad2, c->adj0, c->adj1, c->adjAcc, c->adjRbar, c->adjG, c->adjW, c->adjGhost,
+                    c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
+                    c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -889,7 +940,8 @@ extern "C" int adfvm_primal_step(adfvm_ctx* c, void* u, double dt, const void* S
 namespace {
 template <typename T> int gradients_t(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
     launch_ghosts<T>(c, u, s);
-    launch_grad<T>(c, u, grad, s);
+    launch_grad<T>(c, u, P<T>(c->grad0), s);
+    grad_rows_kernel<T><<<grid_for(c->nIC), BLOCK, 0, s>>>(c->nIC, P<T>(c->grad0), grad);
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
 }
