@@ -79,8 +79,9 @@ SIGNATURES = {
     'adfvm_ctx_destroy': (None, [vp]),
     'adfvm_last_error': (C.c_char_p, [vp]),
     'adfvm_ctx_precision': (C.c_int, [vp]),
-    'adfvm_primal_step': (C.c_int, [vp, vp, C.c_double, vp, vp, vp, vp]),
-    'adfvm_adjoint_step': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp]),
+    'adfvm_primal_step': (C.c_int, [vp, vp, C.c_double, vp, vp, vp, vp, vp]),
+    'adfvm_tape_bytes': (C.c_int64, [vp]),
+    'adfvm_adjoint_step': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp, vp]),
     'adfvm_gradients': (C.c_int, [vp, vp, vp, vp]),
     'adfvm_residual': (C.c_int, [vp, vp, vp, vp, vp]),
     'adfvm_residual_vjp': (C.c_int, [vp, vp, vp, vp, vp]),

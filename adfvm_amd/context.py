@@ -165,18 +165,30 @@ class Context(object):
         return t
 
     # ------------------------------------------------------------ entries --
-    def primal_step(self, u, dt, source=None, dtc_out=None, obj_out=None, stream=None):
+    def tape_bytes(self):
+        return int(self.lib.adfvm_tape_bytes(self.handle))
+
+    def new_tapes(self, n):
+        """n step tapes as one device tensor of shape (n, tape scalars)."""
+        k = self.tape_bytes() // self.precision.itemsize
+        return self.torch.empty((n, k), dtype=self.tdtype, device='cuda')
+
+    def primal_step(self, u, dt, source=None, dtc_out=None, obj_out=None, tape=None, stream=None):
         self._check(u, (5, self.nIC))
         if source is not None:
             self._check(source, (5, self.nIC))
+        if tape is not None:
+            assert tape.is_contiguous() and tape.numel() * tape.element_size() >= self.tape_bytes()
         self._chk(self.lib.adfvm_primal_step(self.handle, _dptr(u), float(dt), _dptr(source), _dptr(dtc_out),
-                                             _dptr(obj_out), _stream(stream)))
+                                             _dptr(obj_out), _dptr(tape), _stream(stream)))
 
-    def adjoint_step(self, u_n, dt, ua, obja=1.0, source=None, source_grad=None, stream=None):
+    def adjoint_step(self, u_n, dt, ua, obja=1.0, source=None, source_grad=None, tape=None, stream=None):
         self._check(u_n, (5, self.nIC))
         self._check(ua, (5, self.nIC))
+        if tape is not None:
+            assert tape.is_contiguous() and tape.numel() * tape.element_size() >= self.tape_bytes()
         self._chk(self.lib.adfvm_adjoint_step(self.handle, _dptr(u_n), float(dt), _dptr(source), _dptr(ua),
-                                              float(obja), _dptr(source_grad), _stream(stream)))
+                                              float(obja), _dptr(source_grad), _dptr(tape), _stream(stream)))
 
     def gradients(self, u, out=None, stream=None):
         self._check(u, (5, self.nIC))

@@ -753,13 +753,26 @@ void stages(int integrator, std::vector<Stage>& st) {
     st.push_back({1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0});
 }
 
+// A step tape holds the forward step's intermediate stage states (all
+// stages but the first, whose state is the step's input) and every stage's
+// gradient records: [(ns-1) x 5 x nIC][ns x GREC x nIC] scalars.
+long tape_scalars(const adfvm_ctx* c, int ns) { return ((long)(ns - 1) * 5 + (long)ns * GREC) * c->nIC; }
+
+template <typename T> void tape_slots(const adfvm_ctx* c, int ns, T* tape, T** states, T** grads) {
+    for (int i = 0; i < ns - 1; i++) states[i] = tape + (long)i * 5 * c->nIC;
+    T* g = tape + (long)(ns - 1) * 5 * c->nIC;
+    for (int i = 0; i < ns; i++) grads[i] = g + (long)i * GREC * c->nIC;
+}
+
 template <typename T>
-int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, double* obj_out, hipStream_t s) {
+int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, double* obj_out, T* tape,
+                  hipStream_t s) {
     std::vector<Stage> st;
     stages(c->integrator, st);
     const int ns = (int)st.size();
     T* bufs[3] = {P<T>(c->u1), P<T>(c->u2), u};
     T* grads[3] = {P<T>(c->grad0), P<T>(c->grad1), P<T>(c->grad2)};
+    if (tape) tape_slots(c, ns, tape, bufs, grads);
     const T* ui = u;
     const int objStage = ns > 1 ? 1 : 0;    // density.py:412-413
     for (int i = 0; i < ns; i++) {
@@ -930,11 +943,11 @@ extern "C" int adfvm_ctx_precision(const adfvm_ctx* c) { return c ? c->prec : 0;
 #define DISPATCH(c, CALL_F, CALL_D) ((c)->prec == 4 ? (CALL_F) : (CALL_D))
 
 extern "C" int adfvm_primal_step(adfvm_ctx* c, void* u, double dt, const void* S, double* dtc_out,
-                                 double* obj_out, void* stream) {
+                                 double* obj_out, void* tape, void* stream) {
     if (!c || !u) return ADFVM_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
-    return DISPATCH(c, primal_step_t<float>(c, (float*)u, dt, (const float*)S, dtc_out, obj_out, s),
-                    primal_step_t<double>(c, (double*)u, dt, (const double*)S, dtc_out, obj_out, s));
+    return DISPATCH(c, primal_step_t<float>(c, (float*)u, dt, (const float*)S, dtc_out, obj_out, (float*)tape, s),
+                    primal_step_t<double>(c, (double*)u, dt, (const double*)S, dtc_out, obj_out, (double*)tape, s));
 }
 
 namespace {

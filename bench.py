@@ -59,11 +59,15 @@ def step_bytes_model(s, nC, nF, nG):
     return 3 * BP / nC, 3 * (BP + BA) / nC
 
 
-def run_precision(case, u0, prec, K, W, dt, torch, profile):
+def run_precision(case, u0, prec, K, W, dt, torch, profile, recompute=False):
+    """K steps forward keeping each step's start state and tape (stage
+    states and gradients) in HBM, then K adjoint steps in reverse.  With
+    recompute, no tapes: each adjoint step re-evaluates its stages."""
     ctx = case.context(prec)
     nC = ctx.nIC
     u = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
     store = torch.empty((K, 5, nC), dtype=ctx.tdtype, device='cuda')
+    tapes = None if recompute else ctx.new_tapes(K)
     ua = torch.zeros_like(u)
 
     def pa_pass(k, prof=False):
@@ -71,10 +75,10 @@ def run_precision(case, u0, prec, K, W, dt, torch, profile):
             ctx.profile_start()
         for i in range(k):
             store[i].copy_(u)
-            ctx.primal_step(u, dt)
+            ctx.primal_step(u, dt, tape=None if tapes is None else tapes[i])
         ua.zero_()
         for i in reversed(range(k)):
-            ctx.adjoint_step(store[i], dt, ua, obja=1.0)
+            ctx.adjoint_step(store[i], dt, ua, obja=1.0, tape=None if tapes is None else tapes[i])
         return ctx.profile_stop() if prof else None
 
     pa_pass(min(W, K))
@@ -170,6 +174,8 @@ def main():
 
     r32 = run_precision(case, u0, np.float32, args.steps, args.warmup, dt, torch, profile=True)
     print('[bench] fp32 P+A %.3fs primal %.3fs' % (r32['t_pa'], r32['t_p']), file=sys.stderr, flush=True)
+    r32r = run_precision(case, u0, np.float32, args.steps, args.warmup, dt, torch, profile=False, recompute=True)
+    print('[bench] fp32 P+A recompute %.3fs' % r32r['t_pa'], file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
         r64 = run_precision(case, u0, np.float64, args.steps, args.warmup, dt, torch, profile=True)
@@ -206,7 +212,8 @@ def main():
         'dtype': 'f32',
         'data': 'synthetic (Taylor-Green initial field)',
         'config': {'workload': 'TGV %d^3 periodic hex, random permutation + RCM, primal+adjoint, SSP-RK3, Roe, '
-                               'fixed dt, start states of all K steps kept in HBM' % args.n,
+                               'fixed dt, start states and step tapes (stage states + gradients) of all K steps '
+                               'kept in HBM' % args.n,
                    'n_cells': nC, 'n_faces': nF, 'dt': dt, 'objective': 'kinetic energy on the stage-1 state'},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': (achieved / (HBM_PEAK / 1e9)) if achieved else None, 'traffic': traffic,
@@ -214,6 +221,7 @@ def main():
         'hbm_frac_step_model': value * bpc_pa32 / HBM_PEAK,
         'bytes_per_cell_update_model': {'fp32_primal': bpc_p32, 'fp32_pa': bpc_pa32, 'fp64_pa': bpc_pa64},
         'primal_cell_updates_per_s_fp32': nC * K / r32['t_p'],
+        'value_fp32_recompute_stages': nC * K / r32r['t_pa'],
         'kernel_ms_fp32': {k: round(v[0] / K, 4) for k, v in prof.items()},
         'finite': r32['finite'],
     }

@@ -189,18 +189,27 @@ int adfvm_ctx_precision(const adfvm_ctx* ctx);
  * RCF.equation, density.py:339-414).  source: device [5][nIC] or NULL.
  * dtc_out (device double, nullable): max_c sum_f (|Un|+a)A/V evaluated on the
  * first intermediate stage, density.py:405-413.  obj_out (device double,
- * nullable): the objective on the same stage, in FP64. */
+ * nullable): the objective on the same stage, in FP64.  tape (device,
+ * adfvm_tape_bytes() bytes, nullable): receives the step's intermediate
+ * stage states and stage gradients for a later adfvm_adjoint_step. */
 int adfvm_primal_step(adfvm_ctx* ctx, void* u, double dt, const void* source,
-                      double* dtc_out, double* obj_out, void* stream);
+                      double* dtc_out, double* obj_out, void* tape, void* stream);
+
+/* Size in bytes of one step's tape (2 stage states + 3 stage gradient
+ * records for SSP-RK3).  Keeping tapes trades HBM capacity for the forward
+ * recomputation the reference's reverse sweep does (apps/adjoint.py:211-217). */
+int64_t adfvm_tape_bytes(const adfvm_ctx* ctx);
 
 /* Reverse of one time step (the primal_grad Function, apps/adjoint.py:94-126,
  * called at :268-291 with dtca = 0).  u_n: device [5][nIC], the primal state
  * at the start of the step (unchanged).  ua: device [5][nIC]; in: adjoint of
  * the step's output, out: adjoint of its input, d(obja*J_n + <ua_in,u_{n+1}>)/du_n.
  * source_grad (device [5][nIC], nullable): the source-term gradient of the
- * same functional is ADDED to it (parameter 'source', apps/adjoint.py:104-106). */
+ * same functional is ADDED to it (parameter 'source', apps/adjoint.py:104-106).
+ * tape (nullable): the tape the forward step from u_n filled; without it the
+ * step's stages are recomputed from u_n. */
 int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* source,
-                       void* ua, double obja, void* source_grad, void* stream);
+                       void* ua, double obja, void* source_grad, const void* tape, void* stream);
 
 /* Operator-level entry points (the hot path's pieces, used by tests). */
 
