@@ -4,8 +4,10 @@ Workload: 3-D Taylor-Green vortex on a periodic 128^3 hex box of side 2*pi,
 stored as an unstructured face list (seeded random cell permutation, then
 reverse Cuthill-McKee), Mach 0.1, rho0 = 1.2, Re = 1600, Roe flux, SSP-RK3,
 fixed dt.  One "step" is one time step forward plus its reverse (adjoint)
-step: K forward steps keep their start states in HBM, then K adjoint steps
-run in reverse (each re-evaluates its RK stages from the stored state).
+step: K forward steps keep their start states and step tapes (intermediate
+stage states and stage gradients) in HBM, then K adjoint steps run in
+reverse from them.  The rate with tapes off (each adjoint step recomputing
+its stages, as the reference's reverse sweep does) is reported beside it.
 
 Reported value: FP32 primal+adjoint cell-updates/s (nCells * K / time).
 Also reported: the same for FP64, primal-only rates, the FP32-vs-FP64
