@@ -1,0 +1,206 @@
+"""RCF: the compressible Navier-Stokes solver of adFVM (adFVM/density.py)
+on the MI355X kernels.
+
+Same configuration keys and defaults as RCF.defaultConfig
+(density.py:22-39).  `compile()` creates the device context (mesh, boundary
+values, physics) and binds `self.map`, a callable with the reference
+primal Function's positional contract (density.py:101-105, called at
+solver.py:312-323):
+
+    rho', rhoU', rhoE', dtc, obj = primal.map(rho, rhoU, rhoE, dt,
+                                              *mesh.getTensor(), *mesh.getScalar(),
+                                              *sources, *bc_values)
+
+(mesh and boundary arguments are accepted and ignored: the context holds
+them).  The host loop, `run`, keeps the state on the device instead.
+"""
+import numpy as np
+
+from . import config, _lib
+from . import BCs
+from . import objectives
+from .solver import Solver
+
+
+def mu_model(mu):
+    """Viscosity spec -> (model, mu0, A, Ts) for the kernels.  Accepts a
+    number, ('sutherland', A, Ts), or a callable mu(T) that is constant or of
+    Sutherland form A*T^1.5/(T+Ts) (RCF's default, density.py:27)."""
+    if mu is None:
+        return (_lib.MU_SUTHERLAND, 0., 1.4792e-06, 116.)
+    if np.isscalar(mu):
+        return (_lib.MU_CONSTANT, float(mu), 0., 0.)
+    if isinstance(mu, tuple) and mu and mu[0] == 'sutherland':
+        return (_lib.MU_SUTHERLAND, 0., float(mu[1]), float(mu[2]))
+    if callable(mu):
+        Ts = np.array([150., 300., 600.])
+        vals = np.array([float(np.asarray(mu(T)).reshape(-1)[0]) for T in Ts])
+        if np.allclose(vals, vals[0], rtol=1e-12, atol=0):
+            return (_lib.MU_CONSTANT, float(vals[0]), 0., 0.)
+        # fit A, S from two points of A T^1.5/(T+S) and check the third
+        r = vals[0] / vals[1] * (Ts[1] ** 1.5 / Ts[0] ** 1.5)   # = (T0+S)^-1 (T1+S)
+        S = (Ts[1] - r * Ts[0]) / (r - 1)
+        A = vals[0] * (Ts[0] + S) / Ts[0] ** 1.5
+        if np.isclose(A * Ts[2] ** 1.5 / (Ts[2] + S), vals[2], rtol=1e-9):
+            return (_lib.MU_SUTHERLAND, 0., float(A), float(S))
+    raise ValueError('viscosity must be constant or Sutherland-form, got %r' % (mu,))
+
+
+class RCF(Solver):
+    defaultConfig = dict(Solver.defaultConfig)
+    defaultConfig.update({
+        'Cp': 1004.5,
+        'gamma': 1.4,
+        'mu': None,                 # None = Sutherland 1.4792e-06*T**1.5/(T+116.)
+        'Pr': 0.7,
+        'CFL': 1.2,
+        'stepFactor': 1.2,
+        'timeIntegrator': 'SSPRK', 'nStages': 3,
+        'riemannSolver': 'eulerRoe',
+        'objectivePLInfo': None,
+        'boundaryRiemannSolver': 'eulerRoe',
+        'readConservative': False,
+        'faceReconstructor': 'secondOrder',   # or 'secondOrderLimited' (clamped; not in the reference)
+    })
+
+    def __init__(self, case, **userConfig):
+        super(RCF, self).__init__(case, **userConfig)
+        self.Cv = self.Cp / self.gamma
+        self.R = self.Cp - self.Cv
+        self.names = ['rho', 'rhoU', 'rhoE']
+        self.dimensions = [(1,), (3,), (1,)]
+        self.boundaries = {'U': {}, 'T': {}, 'p': {}}
+        self.objective = objectives.resolve(self.objective)
+        for key, allowed in (('riemannSolver', ('eulerRoe', 'eulerLaxFriedrichs')),
+                             ('boundaryRiemannSolver', ('eulerRoe', 'eulerLaxFriedrichs')),
+                             ('timeIntegrator', ('SSPRK', 'euler')),
+                             ('faceReconstructor', ('secondOrder', 'secondOrderLimited'))):
+            if getattr(self, key) not in allowed:
+                raise ValueError('%s=%r not supported (%s)' % (key, getattr(self, key), ', '.join(allowed)))
+
+    # ------------------------------------------------------- conversions --
+    def primitive(self, rho, rhoU, rhoE):
+        """adFVM/density.py:162-171 (host arrays)."""
+        U = rhoU / rho
+        E = rhoE / rho
+        e = E - 0.5 * (U * U).sum(-1, keepdims=True)
+        p = (self.gamma - 1) * rho * e
+        T = e * (1. / self.Cv)
+        return U, T, p
+
+    def conservative(self, U, T, p):
+        """adFVM/density.py:173-181 (host arrays)."""
+        e = self.Cv * T
+        rho = p / (e * (self.gamma - 1))
+        rhoE = rho * (e + 0.5 * (U * U).sum(-1, keepdims=True))
+        rhoU = U * rho
+        return rho, rhoU, rhoE
+
+    # ------------------------------------------------------------ fields --
+    def setFields(self, U, T, p, boundary=None):
+        """Initial primitive fields on the internal cells ((nIC,3), (nIC,1),
+        (nIC,1)) and their boundary conditions {field: {patch: spec}}; the
+        role of readFields for generated cases."""
+        if boundary is not None:
+            for k in boundary:
+                self.boundaries[k] = dict(boundary[k])
+        U = np.asarray(U, np.float64).reshape(-1, 3)
+        T = np.asarray(T, np.float64).reshape(-1, 1)
+        p = np.asarray(p, np.float64).reshape(-1, 1)
+        rho, rhoU, rhoE = self.conservative(U, T, p)
+        self.setState(np.concatenate([rho.T, rhoU.T, rhoE.T]))
+        return [rho, rhoU, rhoE]
+
+    def setState(self, u):
+        """Conservative state (5, nIC) as rows rho, rhoU, rhoE."""
+        torch = self._torch()
+        dt = torch.float32 if config.precision == np.float32 else torch.float64
+        self.u = torch.as_tensor(np.ascontiguousarray(u), dtype=dt).to('cuda').contiguous()
+        self.firstRun = False
+
+    def readFields(self, t, suffix=''):
+        """Read U, T, p (+ boundaryField) of time directory t from the case
+        (adFVM/density.py:113-137)."""
+        from . import foam
+        U, T, p, bnd = foam.read_fields(self.mesh, t, suffix)
+        return self.setFields(U, T, p, boundary=bnd)
+
+    def getFields(self):
+        """Host conservative fields [rho, rhoU, rhoE]."""
+        u = self.u.double().cpu().numpy()
+        return [u[0:1].T.copy(), u[1:4].T.copy(), u[4:5].T.copy()]
+
+    def writeFields(self, t):
+        """Write the primitive and conservative fields at time t."""
+        from . import foam
+        rho, rhoU, rhoE = self.getFields()
+        U, T, p = self.primitive(rho, rhoU, rhoE)
+        foam.write_fields(self.mesh, t, {'rho': rho, 'rhoU': rhoU, 'rhoE': rhoE, 'U': U, 'T': T, 'p': p},
+                          self.boundaries)
+
+    def updateSource(self, source, perturb=False):
+        """Source terms (rho, rhoU, rhoE) added to the residual
+        (adFVM/solver.py:116-133)."""
+        torch = self._torch()
+        s = np.concatenate([np.asarray(source[0], np.float64).reshape(-1, 1).T,
+                            np.asarray(source[1], np.float64).reshape(-1, 3).T,
+                            np.asarray(source[2], np.float64).reshape(-1, 1).T]) \
+            if not np.isscalar(source[0]) else np.zeros((5, self.mesh.nInternalCells)) + \
+            np.array([source[0]] + [source[1]] * 3 + [source[2]])[:, None]
+        dt = torch.float32 if config.precision == np.float32 else torch.float64
+        st = torch.as_tensor(s, dtype=dt).to('cuda')
+        if perturb and self.source is not None:
+            self.source += st
+        else:
+            self.source = st.contiguous()
+        if not bool((self.source != 0).any()):
+            self.source = None
+
+    # ----------------------------------------------------------- compile --
+    def physics(self):
+        obj = self.objective
+        ph = dict(gamma=self.gamma, Cp=self.Cp, Pr=self.Pr, mu=mu_model(self.mu),
+                  riemann=_lib.RIEMANN_ROE if self.riemannSolver == 'eulerRoe' else _lib.RIEMANN_LF,
+                  boundaryRiemann=_lib.RIEMANN_ROE if self.boundaryRiemannSolver == 'eulerRoe' else _lib.RIEMANN_LF,
+                  small=config.small_for(config.precision),
+                  integrator=_lib.RK_SSPRK3 if self.timeIntegrator == 'SSPRK' else _lib.RK_EULER,
+                  objective=obj.code if obj else _lib.OBJ_NONE,
+                  limiter=1 if self.faceReconstructor == 'secondOrderLimited' else 0)
+        if obj is not None and obj.patch is not None:
+            ph['objectivePatch'] = obj.patch
+        return ph
+
+    def compile(self):
+        """Create the device context: Function.compile + Function.initialize
+        of the reference (solver.py:61-71)."""
+        from .context import Context
+        specs = BCs.validate(self.mesh, self.boundaries)
+        self.ctx = Context(self.mesh, specs, self.physics(), config.precision)
+        if self.u is not None and self.u.dtype != self.ctx.tdtype:
+            self.u = self.u.to(self.ctx.tdtype).contiguous()
+        self.map = PrimalMap(self)
+        return self
+
+
+class PrimalMap(object):
+    """The primal Function's call contract (density.py:101-105): positional
+    [rho, rhoU, rhoE, dt] + mesh tensors + mesh scalars + sources + boundary
+    values; returns [rho', rhoU', rhoE', dtc (1,1), obj (1,1)]."""
+
+    def __init__(self, solver):
+        self.solver = solver
+
+    def __call__(self, rho, rhoU, rhoE, dt, *rest, **options):
+        s = self.solver
+        torch = s._torch()
+        ctx = s.ctx
+        td = ctx.tdtype
+        nIC = ctx.nIC
+        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=td).to('cuda')
+        u = ctx.pack(dev(np.reshape(rho, (nIC, 1))), dev(np.reshape(rhoU, (nIC, 3))), dev(np.reshape(rhoE, (nIC, 1))))
+        dtc = torch.zeros(1, dtype=torch.float64, device='cuda')
+        obj = torch.zeros(1, dtype=torch.float64, device='cuda')
+        ctx.primal_step(u, float(np.asarray(dt).reshape(-1)[0]), source=s.source, dtc_out=dtc, obj_out=obj)
+        r, rU, rE = ctx.unpack(u)
+        f = lambda t: t.cpu().numpy()
+        return [f(r), f(rU), f(rE), dtc.cpu().numpy().reshape(1, 1), obj.cpu().numpy().reshape(1, 1)]

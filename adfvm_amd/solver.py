@@ -1,0 +1,180 @@
+"""Solver base class and the host time loop (adFVM/solver.py).
+
+The loop keeps the state resident on the GPU as a (5, nInternalCells) tensor
+and calls the library step once per time step, as the reference calls its
+compiled `self.map` (solver.py:312-323).  Host synchronisation happens only
+where the reference's loop needs a host value: the CFL time-step update
+(solver.py:359-366), report and write steps.  The time-integrated objective
+is accumulated in FP64 on the device.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import config
+from .mesh import Mesh
+from .meshgen import MeshData
+
+
+class Solver(object):
+    defaultConfig = {
+        'timeIntegrator': 'euler',
+        'objective': None,
+        'objectiveString': None,
+        'adjoint': False,
+        'dynamicMesh': False,
+        'localTimeStep': False,
+        'fixedTimeStep': False,
+        'stepFactor': 1.0,
+        'postpro': [],
+        'sourceTerms': [],
+        'timeSeriesAppend': '',
+    }
+
+    def __init__(self, case, **userConfig):
+        """case: an OpenFOAM case directory, a built Mesh, or generated
+        MeshData (adfvm_amd.meshgen).  Unknown configuration keys are
+        rejected as in solver.py:37-39 (the class default dict is copied,
+        not mutated)."""
+        full = dict(self.__class__.defaultConfig)
+        for key in userConfig:
+            if key not in full:
+                raise KeyError('unknown solver option %r' % key)
+        full.update(userConfig)
+        for key, v in full.items():
+            setattr(self, key, v)
+        if isinstance(case, Mesh):
+            self.mesh = case
+        elif isinstance(case, MeshData):
+            self.mesh = Mesh.from_data(case)
+        else:
+            self.mesh = Mesh.create(case)
+        caseDir = self.mesh.case or ''
+        self.resultFile = caseDir + 'objective.txt'
+        self.statusFile = caseDir + 'status.json'
+        self.timeSeriesFile = caseDir + 'timeSeries{}.txt'.format(self.timeSeriesAppend)
+        self.timeStepFile = caseDir + 'timeSteps.txt'
+        self.stage = 0
+        self.firstRun = True
+        self.ctx = None
+        self.u = None                      # device state (5, nIC)
+        self.source = None                 # device source (5, nIC) or None
+        return
+
+    # ------------------------------------------------------------- status --
+    def readStatusFile(self):
+        with open(self.statusFile) as f:
+            return json.load(f)
+
+    def writeStatusFile(self, data):
+        with open(self.statusFile, 'w') as f:
+            json.dump(data, f)
+
+    def removeStatusFile(self):
+        try:
+            os.remove(self.statusFile)
+        except OSError:
+            pass
+
+    # --------------------------------------------------------- host loop --
+    def run(self, endTime=np.inf, writeInterval=config.LARGE, reportInterval=1, startTime=0.0, dt=1e-3,
+            nSteps=config.LARGE, startIndex=0, result=0., mode='simulation', source=None, perturbation=None,
+            avgStart=0, tapes=None):
+        """The reference's Solver.run (solver.py:210-411).
+
+        dt: a number, or an array of per-step dt (replay of a logged run).
+        mode 'forward' returns the list of device states u_n at the start of
+        each step (and, with `tapes`, fills tapes[n] with step n's tape).
+        source(fields, mesh, t) -> (rho, rhoU, rhoE) source terms;
+        perturbation = (parameters, perturb) adds perturb(...) to the
+        'source' parameter for the run and reverts it after
+        (solver.py:243-277,406-407).
+        Returns the time-integrated objective (FP64), or the states."""
+        torch = self._torch()
+        if self.u is None:
+            raise RuntimeError('no fields: call readFields()/setFields() first')
+        if self.ctx is None:
+            self.compile()
+        mesh = self.mesh
+        t = startTime
+        dts = dt
+        timeIndex = startIndex
+        if isinstance(dts, np.ndarray):
+            dt = float(dts[timeIndex - startIndex]) if len(dts) > 0 else 0.0
+        fixed = self.fixedTimeStep or isinstance(dts, np.ndarray)
+        timeSeries, timeSteps, solutions = [], [], []
+        if source is not None:
+            self.updateSource(source(self.getFields(), mesh, t))
+        if perturbation:
+            self._perturb(perturbation, t, revert=False)
+
+        dev = 'cuda'
+        obj = torch.zeros(1, dtype=torch.float64, device=dev)
+        dtc = torch.zeros(1, dtype=torch.float64, device=dev)
+        acc = torch.zeros(1, dtype=torch.float64, device=dev)
+        series = []
+        start = time.time()
+
+        def iterate(t, i):
+            return t < endTime and i < nSteps
+
+        def updateTime(t0, dt):
+            return round(t0 + dt, 12)
+
+        while iterate(t, timeIndex):
+            report = ((timeIndex + 1) % reportInterval == 0)
+            write = ((timeIndex + 1) % writeInterval == 0) or not iterate(updateTime(t, dt), timeIndex + 1)
+            if mode == 'forward':
+                solutions.append(self.u.clone())
+            tape = tapes[timeIndex - startIndex] if tapes is not None else None
+            self.ctx.primal_step(self.u, dt, source=self.source, dtc_out=dtc, obj_out=obj, tape=tape)
+            timeSteps.append([t, dt])
+            timeIndex += 1
+            t = updateTime(t, dt)
+            if timeIndex > avgStart:
+                acc += obj
+            series.append(obj.clone())
+            if report:
+                if not bool(torch.isfinite(self.u).all()):
+                    raise FloatingPointError('NaN/Inf in the state at step %d (t=%g)' % (timeIndex, t))
+            if isinstance(dts, np.ndarray):
+                dt = float(dts[timeIndex - startIndex]) if timeIndex - startIndex < len(dts) else dt
+            elif not fixed:
+                dt = min(2 * self.CFL / dtc.item(), dt * self.stepFactor, endTime - t)
+            if write and mode != 'forward':
+                if mode in ('orig', 'simulation'):
+                    self.writeFields(t)
+                if mode == 'orig':
+                    with open(self.timeStepFile, 'ab') as f:
+                        np.savetxt(f, np.array(timeSteps))
+                if mode in ('orig', 'perturb'):
+                    with open(self.timeSeriesFile, 'ab') as f:
+                        np.savetxt(f, torch.cat(series).cpu().numpy())
+                series, timeSteps = [], []
+                self.writeStatusFile([timeIndex, t, dt, result + acc.item()])
+        if perturbation:
+            self._perturb(perturbation, t, revert=True)
+        self.lastTime, self.lastDt, self.wallTime = t, dt, time.time() - start
+        if mode == 'forward':
+            solutions.append(self.u.clone())     # nSteps+1 states, as solver.py:234-241,376-382
+            return solutions
+        return result + acc.item()
+
+    def _perturb(self, perturbation, t, revert):
+        parameters, perturb = perturbation
+        values = perturb(self.getFields(), self.mesh, t)
+        if not isinstance(values, list) or (len(parameters) == 1 and len(values) > 1):
+            values = [values]
+        for param, value in zip(parameters, values):
+            if param != 'source':
+                raise NotImplementedError('perturbation of %r (only the source parameter is supported)' % (param,))
+            sign = -1. if revert else 1.
+            self.updateSource([sign * np.asarray(v, np.float64) for v in value], perturb=True)
+
+    def _torch(self):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError('adfvm_amd needs a HIP device')
+        return torch

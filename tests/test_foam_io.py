@@ -1,0 +1,86 @@
+"""OpenFOAM case I/O (adfvm_amd.foam): the shockTube case files of the
+reference (binary fields, boundary dictionary) and write/read round trips."""
+import os
+import shutil
+
+import numpy as np
+
+from adfvm_amd import foam, meshgen
+from adfvm_amd.mesh import Mesh
+
+HERE = os.path.dirname(__file__)
+SHOCKTUBE = os.path.join(HERE, 'golden', 'shockTube')
+
+
+def test_read_reference_boundary():
+    b = foam.read_boundary(os.path.join(SHOCKTUBE, 'constant', 'polyMesh', 'boundary'))
+    assert list(b) == ['sides', 'empty']
+    assert (b['sides']['type'], b['sides']['nFaces'], b['sides']['startFace']) == ('patch', 2, 499)
+    assert (b['empty']['type'], b['empty']['nFaces'], b['empty']['startFace']) == ('empty', 2000, 501)
+
+
+def test_generated_shocktube_matches_reference_boundary():
+    md = meshgen.shock_tube(500, 10.0, 2.0, origin=-5.0)
+    ref = foam.read_boundary(os.path.join(SHOCKTUBE, 'constant', 'polyMesh', 'boundary'))
+    assert list(md.boundary) == list(ref)
+    for name in ref:
+        assert md.boundary[name]['nFaces'] == ref[name]['nFaces']
+        assert md.boundary[name]['startFace'] == ref[name]['startFace']
+        assert md.boundary[name]['type'] == ref[name]['type']
+    assert len(md.neighbour) == 499 and md.points[:, 0].min() == -5.0 and md.points[:, 0].max() == 5.0
+
+
+def test_read_reference_binary_fields():
+    U, bU = foam.read_field(SHOCKTUBE, 0, 'U', 500, 3)
+    T, bT = foam.read_field(SHOCKTUBE, 0, 'T', 500, 1)
+    p, bp = foam.read_field(SHOCKTUBE, 0, 'p', 500, 1)
+    assert (U == 0).all()
+    assert (p[:250] == 1e5).all() and (p[250:] == 1e4).all()
+    R = 1004.5 - 1004.5 / 1.4
+    rho = p / (R * T)
+    assert np.allclose(rho[:250], 1.0, rtol=1e-5) and np.allclose(rho[250:], 0.125, rtol=1e-5)
+    assert bp == {'sides': {'type': 'zeroGradient'}, 'empty': {'type': 'empty'}}
+    assert bU['empty'] == {'type': 'empty'}
+
+
+def test_polymesh_round_trip(tmp_path):
+    for md in (meshgen.periodic_box(3), meshgen.o_grid(12, 4), meshgen.shock_tube(7, 1.0)):
+        d = str(tmp_path / 'case')
+        foam.write_polymesh(md, d)
+        back = foam.read_polymesh(d)
+        assert np.array_equal(back.points, md.points)
+        assert np.array_equal(back.faces, md.faces)
+        assert np.array_equal(back.owner, md.owner)
+        assert np.array_equal(back.neighbour, md.neighbour)
+        assert list(back.boundary) == list(md.boundary)
+        for name, b in md.boundary.items():
+            for k in ('type', 'nFaces', 'startFace'):
+                assert back.boundary[name][k] == b[k]
+            if 'neighbourPatch' in b:
+                assert back.boundary[name]['neighbourPatch'] == b['neighbourPatch']
+        shutil.rmtree(d)
+
+
+def test_field_round_trip(tmp_path):
+    md = meshgen.box((3, 2, 2))
+    d = str(tmp_path / 'case')
+    foam.write_polymesh(md, d)
+    mesh = Mesh.create(d)
+    rng = np.random.default_rng(0)
+    U = rng.standard_normal((12, 3))
+    T = np.full((12, 1), 300.)
+    p = 1e5 + rng.standard_normal((12, 1))
+    bnd = {'U': {'walls': {'type': 'fixedValue', 'value': np.zeros((mesh.boundary['walls']['nFaces'], 3))}},
+           'T': {'walls': {'type': 'zeroGradient'}},
+           'p': {'walls': {'type': 'zeroGradient'}}}
+    foam.write_fields(mesh, 0.00125, {'U': U, 'T': T, 'p': p}, bnd)
+    assert os.path.isdir(os.path.join(d, '0.00125000000'))
+    U2, T2, p2, b2 = foam.read_fields(mesh, 0.00125)
+    assert np.array_equal(U2, U) and np.array_equal(T2, T) and np.array_equal(p2, p)
+    assert b2['U']['walls']['type'] == 'fixedValue' and b2['U']['walls']['value'].startswith('uniform')
+    assert b2['p']['walls'] == {'type': 'zeroGradient'}
+
+
+def test_time_string():
+    assert foam.time_string(0) == '0' and foam.time_string(2.0) == '2'
+    assert foam.time_string(0.006341272) == '0.00634127200'
