@@ -1,0 +1,273 @@
+"""Reverse sweep of an unsteady run: the reference's Adjoint solver
+(apps/adjoint.py:30-404) over the device kernels.
+
+Two layers:
+
+  sweep(ctx, u0, dts, ...)   in-process reverse of nSteps steps from the
+                             device state u0 with a checkpoint schedule
+                             (store_all / uniform / revolve, see
+                             adfvm_amd.checkpoint); step tapes keep each
+                             re-run segment's stage states and gradients in
+                             HBM so that no step is recomputed twice.
+  Adjoint(primal)            the case-file workflow of apps/adjoint.py: the
+                             primal's written time directories are the
+                             checkpoints, segments of writeInterval steps are
+                             re-run and reversed from the last, the source
+                             sensitivity of every perturbation is accumulated
+                             in FP64, status/sensTimeSeries/energyTimeSeries
+                             files and the adjoint fields are written.
+
+Adjoint fields are gradients of the objective with respect to the
+conservative state, i.e. "times volume" as the reference keeps them
+internally (apps/adjoint.py:185-189); they are divided by the cell volumes
+when written.
+"""
+import os
+import time
+
+import numpy as np
+
+from . import checkpoint
+from . import foam
+
+
+class DeviceStepper(object):
+    """The checkpoint.stepper interface over a Context.
+
+    dts[n] is step n's dt; obja(n) its objective weight; ua the adjoint
+    state (updated in place); source_grad, if given, accumulates the source
+    gradient; after_reverse(n, u_n) runs after step n is reversed."""
+
+    def __init__(self, ctx, dts, ua, source=None, source_grad=None, obja=1.0, tapes=True, after_reverse=None):
+        self.ctx = ctx
+        self.dts = dts
+        self.ua = ua
+        self.source = source
+        self.source_grad = source_grad
+        self.obja = obja if callable(obja) else (lambda n, w=float(obja): w)
+        self.tapes = tapes
+        self.after_reverse = after_reverse
+        self.forwards = 0
+
+    def clone(self, u):
+        return u.clone()
+
+    def forward(self, u, n, keep):
+        tape = self.ctx.new_tapes(1)[0] if (keep and self.tapes) else None
+        self.ctx.primal_step(u, self.dts[n], source=self.source, tape=tape)
+        self.forwards += 1
+        return tape
+
+    def reverse(self, u_n, n, tape):
+        self.ctx.adjoint_step(u_n, self.dts[n], self.ua, obja=self.obja(n), source=self.source,
+                              source_grad=self.source_grad, tape=tape)
+        if self.after_reverse is not None:
+            self.after_reverse(n, u_n)
+
+
+def sweep(ctx, u0, dts, schedule='store_all', interval=None, snapshots=None, obja=1.0, ua=None, source=None,
+          source_grad=None, tapes=True, after_reverse=None, checkpoints=None):
+    """Reverse nSteps = len(dts) steps from the device state u0 (unchanged).
+
+    Returns ua = d/du0 of sum_n obja(n) J_n + <ua_in, u_N> (ua_in = the given
+    ua, zero by default), the objective J_n evaluated inside step n as the
+    primal step does.  source_grad (device (5, nIC)) accumulates the source
+    gradient of the same functional.  schedule: 'store_all', 'uniform'
+    (interval steps per segment; `checkpoints` may hold segment starts
+    n -> state) or 'revolve' (`snapshots` extra states)."""
+    n = len(dts)
+    if ua is None:
+        ua = ctx.state(0.)
+    st = DeviceStepper(ctx, dts, ua, source, source_grad, obja, tapes, after_reverse)
+    if schedule == 'store_all':
+        checkpoint.store_all(st, u0, 0, n)
+    elif schedule == 'uniform':
+        checkpoint.uniform(st, u0, 0, n, int(interval), checkpoints)
+    elif schedule == 'revolve':
+        checkpoint.revolve(st, u0, 0, n, int(snapshots))
+    else:
+        raise ValueError('unknown schedule %r' % (schedule,))
+    sweep.last_forward_steps = st.forwards
+    return ua
+
+
+def initial_velocity_gradient(ua, u0):
+    """Chain rule through the conservative map at fixed rho and T for a
+    perturbation of the initial velocity: with rhoU = rho U and
+    rhoE = rho (Cv T + |U|^2/2), d/dU = rho (ua_rhoU + ua_rhoE U).  Returns
+    (3, nIC) in FP64."""
+    a = ua.double()
+    u = u0.double()
+    rho = u[0]
+    U = u[1:4] / rho
+    return rho * (a[1:4] + a[4] * U)
+
+
+class Adjoint(object):
+    """apps/adjoint.py's Adjoint solver for a primal RCF whose orig run wrote
+    its checkpoints (time directories every writeInterval steps) and its
+    time-step log."""
+
+    def __init__(self, primal, scaling=0.0):
+        self.primal = primal
+        self.mesh = primal.mesh
+        self.names = [name + 'a' for name in primal.names]
+        self.dimensions = primal.dimensions
+        self.statusFile = primal.statusFile
+        self.sensTimeSeriesFile = self.mesh.case + 'sensTimeSeries.txt'
+        self.energyTimeSeriesFile = self.mesh.case + 'energyTimeSeries.txt'
+        self.scaling = scaling
+        self.forceReadFields = False
+        self.homogeneousAdjoint = False
+        self.ua = None
+
+    # ------------------------------------------------------------- fields --
+    def createFields(self):
+        self.ua = self.primal.ctx.state(0.)
+        return self.ua
+
+    def _volumes(self):
+        torch = self.primal._torch()
+        V = torch.as_tensor(self.mesh.volumes.reshape(-1), dtype=self.primal.ctx.tdtype).to('cuda')
+        return V
+
+    def readFields(self, t):
+        """Adjoint fields of time t, times the cell volumes (apps/adjoint.py:
+        185-189)."""
+        nIC = self.mesh.nInternalCells
+        rows = []
+        for name, (k,) in zip(self.names, self.dimensions):
+            v = foam.read_field(self.mesh.caseDir, t, name, nIC, k)[0]
+            rows.append(v.T)
+        torch = self.primal._torch()
+        ua = torch.as_tensor(np.concatenate(rows), dtype=self.primal.ctx.tdtype).to('cuda')
+        self.ua = (ua * self._volumes()).contiguous()
+        return self.ua
+
+    def writeFields(self, t):
+        a = (self.ua / self._volumes()).double().cpu().numpy()
+        fields = {self.names[0]: a[0:1].T, self.names[1]: a[1:4].T, self.names[2]: a[4:5].T}
+        foam.write_fields(self.mesh, t, fields, {})
+
+    def readStatusFile(self):
+        return self.primal.readStatusFile()
+
+    def writeStatusFile(self, data):
+        self.primal.writeStatusFile(data)
+
+    def removeStatusFile(self):
+        self.primal.removeStatusFile()
+
+    # ----------------------------------------------------------- the run --
+    def initPrimalData(self, nSteps, startTime, dt, nPerturb):
+        """(result, firstCheckpoint) from the status file and the primal's
+        time-step log, (nSteps+1, 2) with the final time appended
+        (apps/adjoint.py:146-172)."""
+        if os.path.exists(self.statusFile):
+            firstCheckpoint, result = self.readStatusFile()
+        else:
+            firstCheckpoint, result = 0, [0.] * nPerturb
+        if not os.path.exists(self.primal.timeStepFile):
+            if not self.primal.fixedTimeStep:
+                raise RuntimeError('no time-step log %s and the primal time step is not fixed'
+                                   % self.primal.timeStepFile)
+            timeSteps = np.stack([startTime + np.arange(nSteps) * dt, np.full(nSteps, dt)], 1)
+        else:
+            timeSteps = np.loadtxt(self.primal.timeStepFile, ndmin=2)
+        if timeSteps.shape != (nSteps, 2):
+            raise RuntimeError('time-step log has shape %s, expected (%d, 2)' % (timeSteps.shape, nSteps))
+        timeSteps = np.concatenate([timeSteps, [[np.sum(timeSteps[-1]).round(12), 0.]]])
+        return (result, firstCheckpoint), (timeSteps,)
+
+    def run(self, checkpointData, primalData, nSteps, writeInterval, perturb=(), parameters=('source',),
+            source=None, sampleInterval=1, reportInterval=1, avgStart=0, runCheckpoints=2 ** 30, tapes=True,
+            verbose=False):
+        """The reverse sweep of apps/adjoint.py:176-404 for the 'source'
+        parameter; returns the (unaveraged) sensitivities, one per
+        perturbation."""
+        if parameters and list(parameters)[0] != 'source':
+            raise NotImplementedError('parameter %r (only the source parameter is supported)' % (parameters[0],))
+        primal = self.primal
+        ctx = primal.ctx
+        torch = primal._torch()
+        mesh = self.mesh
+        result, firstCheckpoint = checkpointData
+        result = list(result)
+        (timeSteps,) = primalData
+        startTime = timeSteps[nSteps - firstCheckpoint * writeInterval][0]
+        if firstCheckpoint > 0 or self.forceReadFields:
+            self.readFields(startTime)
+        elif self.ua is None:
+            self.createFields()
+        ua = self.ua
+
+        perturbations = []
+        for p in perturb:
+            values = p(None, mesh, 0)
+            values = list(values) if isinstance(values, (list, tuple)) else [values]
+            vals = [np.asarray(v, np.float64).reshape(mesh.nInternalCells, -1) for v in values]
+            d = torch.as_tensor(np.concatenate([v.T for v in vals]), dtype=ctx.tdtype).to('cuda').contiguous()
+            perturbations.append(d)
+        sgrad = ctx.state(0.)
+        dsum = torch.zeros(1, dtype=torch.float64, device='cuda')
+        energy = torch.zeros(1, dtype=torch.float64, device='cuda')
+        sensTimeSeries, energyTimeSeries = [], []
+        obja = 0. if self.homogeneousAdjoint else 1.
+
+        totalCheckpoints = nSteps // writeInterval
+        nCheckpoints = min(firstCheckpoint + runCheckpoints, totalCheckpoints)
+        cp = firstCheckpoint
+        t = startTime
+        while cp < nCheckpoints:
+            primalIndex = nSteps - (cp + 1) * writeInterval
+            t = timeSteps[primalIndex, 0]
+            dts = timeSteps[primalIndex:primalIndex + writeInterval, 1]
+            primal.readFields(t)
+            if source is not None:
+                primal.updateSource(source(primal.getFields(), mesh, 0))
+            # forward re-run of the segment (apps/adjoint.py:217), keeping
+            # each step's start state and tape
+            start = time.time()
+            u = primal.u
+            states, tps = [], []
+            for k in range(writeInterval):
+                states.append(u.clone())
+                tps.append(ctx.new_tapes(1)[0] if tapes else None)
+                ctx.primal_step(u, dts[k], source=primal.source, tape=tps[-1])
+            if cp == 0:
+                self.writeFields(timeSteps[-1][0])
+            ctx.adjoint_energy(ua, u, out=energy)
+            energyTimeSeries.append(energy.item() ** 0.5)
+            for r in range(1, writeInterval + 1):              # reverse count, apps/adjoint.py:244-249
+                k = writeInterval - r
+                ctx.adjoint_step(states[k], dts[k], ua, obja=obja, source=primal.source, source_grad=sgrad,
+                                 tape=tps[k])
+                if r % reportInterval == 0:
+                    ctx.adjoint_energy(ua, states[k], out=energy)
+                    energyTimeSeries.append(energy.item() ** 0.5)
+                states[k] = tps[k] = None
+                if r % sampleInterval == 0:
+                    sens = []
+                    for d in perturbations:
+                        ctx.dot(sgrad, d, out=dsum)
+                        sens.append(dsum.item())
+                    sgrad.zero_()
+                    if (nSteps - (primalIndex + k)) > avgStart:
+                        for i, sv in enumerate(sens):
+                            result[i] += sv
+                    sensTimeSeries.extend([[sv / sampleInterval for sv in sens]] * sampleInterval)
+            if verbose:
+                print('adjoint segment %d/%d: %d steps in %.3f s' % (cp + 1, totalCheckpoints, writeInterval,
+                                                                   time.time() - start))
+            cp += 1
+            self.writeStatusFile([cp, result])
+            with open(self.sensTimeSeriesFile, 'ab') as f:
+                np.savetxt(f, np.array(sensTimeSeries).reshape(len(sensTimeSeries), -1))
+            with open(self.energyTimeSeriesFile, 'ab') as f:
+                np.savetxt(f, np.array(energyTimeSeries))
+            sensTimeSeries, energyTimeSeries = [], []
+            self.writeFields(t)
+        self.finished = cp >= totalCheckpoints
+        if self.finished:
+            self.removeStatusFile()
+        return result

@@ -120,9 +120,24 @@ class RCF(Solver):
 
     def readFields(self, t, suffix=''):
         """Read U, T, p (+ boundaryField) of time directory t from the case
-        (adFVM/density.py:113-137)."""
+        (adFVM/density.py:113-137).  The conservative fields rho, rhoU, rhoE
+        are read instead of converting U, T, p when readConservative is set,
+        or when t > 0 and the directory holds them (directories this solver
+        wrote): a restart or checkpoint then reproduces the state exactly."""
+        import os
         from . import foam
         U, T, p, bnd = foam.read_fields(self.mesh, t, suffix)
+        d = foam.time_dir(self.mesh.caseDir, t)
+        names = [n + suffix for n in self.names]
+        if self.readConservative or (t != 0 and all(os.path.exists(os.path.join(d, n)) for n in names)):
+            nIC = self.mesh.nInternalCells
+            rho = foam.read_field(self.mesh.caseDir, t, names[0], nIC, 1)[0]
+            rhoU = foam.read_field(self.mesh.caseDir, t, names[1], nIC, 3)[0]
+            rhoE = foam.read_field(self.mesh.caseDir, t, names[2], nIC, 1)[0]
+            for k in bnd:
+                self.boundaries[k] = dict(bnd[k])
+            self.setState(np.concatenate([rho.T, rhoU.T, rhoE.T]))
+            return [rho, rhoU, rhoE]
         return self.setFields(U, T, p, boundary=bnd)
 
     def getFields(self):
@@ -135,8 +150,8 @@ class RCF(Solver):
         from . import foam
         rho, rhoU, rhoE = self.getFields()
         U, T, p = self.primitive(rho, rhoU, rhoE)
-        foam.write_fields(self.mesh, t, {'rho': rho, 'rhoU': rhoU, 'rhoE': rhoE, 'U': U, 'T': T, 'p': p},
-                          self.boundaries)
+        foam.write_fields(self.mesh, t, {'U': U, 'T': T, 'p': p}, self.boundaries, self.mesh.defaultBoundary)
+        foam.write_fields(self.mesh, t, {'rho': rho, 'rhoU': rhoU, 'rhoE': rhoE}, {})
 
     def updateSource(self, source, perturb=False):
         """Source terms (rho, rhoU, rhoE) added to the residual

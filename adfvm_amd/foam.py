@@ -227,10 +227,10 @@ def _fmt_value(v, ncomp):
     return 'nonuniform List<%s> %d\n(\n%s\n)\n' % (cls, len(a), body)
 
 
-def write_fields(mesh, t, fields, boundaries):
-    """Write each field of {name: (nIC, k) array} to the time directory,
-    boundaryField taken from the solver's boundary specs (U, T, p) or
-    calculated."""
+def write_fields(mesh, t, fields, boundaries, default=None):
+    """Write each field of {name: (nIC, k) array} to the time directory;
+    a patch's boundaryField entry is boundaries[name][patch], else
+    default[patch] (the mesh's calculated boundary by default)."""
     d = time_dir(mesh.caseDir, t)
     os.makedirs(d, exist_ok=True)
     for name, arr in fields.items():
@@ -241,7 +241,7 @@ def write_fields(mesh, t, fields, boundaries):
                  '', 'boundaryField', '{']
         specs = boundaries.get(name, {})
         for patch in mesh.boundary:
-            spec = specs.get(patch, mesh.calculatedBoundary[patch])
+            spec = specs.get(patch, (default or mesh.calculatedBoundary)[patch])
             lines.append('    %s\n    {' % patch)
             lines.append('        type            %s;' % spec['type'])
             for k, v in spec.items():

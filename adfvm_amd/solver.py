@@ -14,6 +14,7 @@ import time
 import numpy as np
 
 from . import config
+from . import foam
 from .mesh import Mesh
 from .meshgen import MeshData
 
@@ -84,7 +85,8 @@ class Solver(object):
             avgStart=0, tapes=None):
         """The reference's Solver.run (solver.py:210-411).
 
-        dt: a number, or an array of per-step dt (replay of a logged run).
+        dt: a number, or an array of per-step dt indexed by the absolute
+        step index (replay of a logged run).
         mode 'forward' returns the list of device states u_n at the start of
         each step (and, with `tapes`, fills tapes[n] with step n's tape).
         source(fields, mesh, t) -> (rho, rhoU, rhoE) source terms;
@@ -93,6 +95,8 @@ class Solver(object):
         (solver.py:243-277,406-407).
         Returns the time-integrated objective (FP64), or the states."""
         torch = self._torch()
+        if self.mesh.caseDir and os.path.isdir(foam.time_dir(self.mesh.caseDir, startTime)):
+            self.readFields(startTime)          # solver.py:217, for cases on disk
         if self.u is None:
             raise RuntimeError('no fields: call readFields()/setFields() first')
         if self.ctx is None:
@@ -102,7 +106,7 @@ class Solver(object):
         dts = dt
         timeIndex = startIndex
         if isinstance(dts, np.ndarray):
-            dt = float(dts[timeIndex - startIndex]) if len(dts) > 0 else 0.0
+            dt = float(dts[timeIndex])          # absolute step index, solver.py:226-227
         fixed = self.fixedTimeStep or isinstance(dts, np.ndarray)
         timeSeries, timeSteps, solutions = [], [], []
         if source is not None:
@@ -140,7 +144,7 @@ class Solver(object):
                 if not bool(torch.isfinite(self.u).all()):
                     raise FloatingPointError('NaN/Inf in the state at step %d (t=%g)' % (timeIndex, t))
             if isinstance(dts, np.ndarray):
-                dt = float(dts[timeIndex - startIndex]) if timeIndex - startIndex < len(dts) else dt
+                dt = float(dts[timeIndex]) if timeIndex < len(dts) else dt
             elif not fixed:
                 dt = min(2 * self.CFL / dtc.item(), dt * self.stepFactor, endTime - t)
             if write and mode != 'forward':

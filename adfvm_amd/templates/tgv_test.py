@@ -1,0 +1,50 @@
+"""Taylor-Green vortex case file: kinetic-energy objective, Gaussian source
+perturbation (the shape of templates/cylinder_test.py's perturb).
+
+Environment: ADFVM_CASE_DIR (default cases/tgv<N>), ADFVM_TGV_N (default 12).
+"""
+import os
+
+import numpy as np
+
+from adfvm_amd import cases
+from adfvm_amd.density import RCF
+from adfvm_amd.objectives import kinetic_energy
+
+n = int(os.environ.get('ADFVM_TGV_N', '12'))
+setup = None
+
+
+def _make():
+    global setup
+    setup = cases.taylor_green(n, seed=0)
+    return setup
+
+
+caseDir = cases.ensure_case(os.environ.get('ADFVM_CASE_DIR', 'cases/tgv%d' % n), _make)
+setup = setup or cases.taylor_green(n, seed=0)
+
+primal = RCF(caseDir,
+             mu=setup.mu,
+             objective=kinetic_energy(),
+             fixedTimeStep=True,
+             )
+
+
+def perturb(fields, mesh, t):
+    mid = np.array([np.pi, np.pi, np.pi])
+    r2 = ((mesh.cellCentres[:mesh.nInternalCells] - mid) ** 2).sum(1, keepdims=True)
+    G = 1e-3 * np.exp(-r2 / 0.5)
+    rho = G
+    rhoU = np.zeros((mesh.nInternalCells, 3))
+    rhoU[:, 0] += G.flatten() * 100
+    rhoE = G * 2e5
+    return rho, rhoU, rhoE
+
+
+parameters = 'source'
+
+nSteps = 20
+writeInterval = 10
+startTime = 0.0
+dt = 0.5 * (2 * np.pi / n) / (setup.params['U0'] + setup.params['c0'])

@@ -152,6 +152,9 @@ def main():
     ap.add_argument('--l2-steps', type=int, default=100)
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-recompute', action='store_true', help='skip the tape-less variant')
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='only run a device-to-device copy of MIB MiB five times (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)' % args.gpus,
@@ -161,6 +164,15 @@ def main():
     from tests import casefactory as cf
     assert torch.cuda.is_available(), 'bench.py needs the GPU'
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'calibrate_copy_bytes_read': n * 4, 'bytes_written': n * 4, 'launches': 5}))
+        return
 
     t0 = time.time()
     case, params = cf.tgv(args.n, seed=0)
@@ -176,13 +188,16 @@ def main():
 
     r32 = run_precision(case, u0, np.float32, args.steps, args.warmup, dt, torch, profile=True)
     print('[bench] fp32 P+A %.3fs primal %.3fs' % (r32['t_pa'], r32['t_p']), file=sys.stderr, flush=True)
-    r32r = run_precision(case, u0, np.float32, args.steps, args.warmup, dt, torch, profile=False, recompute=True)
-    print('[bench] fp32 P+A recompute %.3fs' % r32r['t_pa'], file=sys.stderr, flush=True)
+    r32r = None
+    if not args.no_recompute:
+        r32r = run_precision(case, u0, np.float32, args.steps, args.warmup, dt, torch, profile=False,
+                             recompute=True)
+        print('[bench] fp32 P+A recompute %.3fs' % r32r['t_pa'], file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
         r64 = run_precision(case, u0, np.float64, args.steps, args.warmup, dt, torch, profile=True)
         print('[bench] fp64 P+A %.3fs primal %.3fs' % (r64['t_pa'], r64['t_p']), file=sys.stderr, flush=True)
-    l2 = l2_error(case, u0, args.l2_steps, dt, torch) if not args.no_fp64 else None
+    l2 = l2_error(case, u0, args.l2_steps, dt, torch) if args.l2_steps > 0 else None
     print('[bench] L2 %s' % l2, file=sys.stderr, flush=True)
 
     K = args.steps
@@ -223,7 +238,7 @@ def main():
         'hbm_frac_step_model': value * bpc_pa32 / HBM_PEAK,
         'bytes_per_cell_update_model': {'fp32_primal': bpc_p32, 'fp32_pa': bpc_pa32, 'fp64_pa': bpc_pa64},
         'primal_cell_updates_per_s_fp32': nC * K / r32['t_p'],
-        'value_fp32_recompute_stages': nC * K / r32r['t_pa'],
+        'value_fp32_recompute_stages': nC * K / r32r['t_pa'] if r32r else None,
         'kernel_ms_fp32': {k: round(v[0] / K, 4) for k, v in prof.items()},
         'finite': r32['finite'],
     }
@@ -232,8 +247,9 @@ def main():
         out.update({'value_fp64': v64, 'fp32_over_fp64': value / v64,
                     'hbm_frac_step_model_fp64': v64 * bpc_pa64 / HBM_PEAK,
                     'primal_cell_updates_per_s_fp64': nC * K / r64['t_p'],
-                    'kernel_ms_fp64': {k: round(v[0] / K, 4) for k, v in r64['prof'].items()},
-                    'l2_rel_err_fp32_vs_fp64': l2, 'l2_steps': args.l2_steps})
+                    'kernel_ms_fp64': {k: round(v[0] / K, 4) for k, v in r64['prof'].items()}})
+    if l2 is not None:
+        out.update({'l2_rel_err_fp32_vs_fp64': l2, 'l2_steps': args.l2_steps})
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline()
     print(json.dumps(out))

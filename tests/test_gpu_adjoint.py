@@ -45,14 +45,17 @@ def scaled_rbar(case, u, seed):
 
 def adjoint_case(name):
     """The primal cases, with the shock tube's exactly uniform (U = 0) halves
-    made generic by 1% relative noise: at (near-)ties the limiter's clamp
-    branch is decided by rounding in near-zero gradients, a subgradient choice
-    no two evaluation orders (or precisions) share; 1% keeps the clamp
-    margins far above FP32 rounding while both branches still occur."""
+    made generic by 1% relative noise and a random O(1 m/s) velocity: at
+    (near-)ties the limiter's clamp branch is decided by rounding in near-zero
+    gradients, and a uniform velocity puts the Roe entropy fix's
+    |Un_L - Un_R| on its kink at 0 -- subgradient choices no two evaluation
+    orders (or precisions) share.  1% keeps the clamp margins far above FP32
+    rounding while both branches still occur."""
     case, u = CASES[name]()
     if name == 'sod':
-        u = u * (1 + 1e-2 * np.random.default_rng(5).normal(size=u.shape))
-        u[1] += 1e-2 * u[0]
+        rng = np.random.default_rng(5)
+        u = u * (1 + 1e-2 * rng.normal(size=u.shape))
+        u[1] += u[0] * rng.normal(size=u.shape[1])
     return case, u
 
 
