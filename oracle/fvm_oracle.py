@@ -12,6 +12,31 @@ face order, then neighbour faces), a fixed order; the reference leaves the
 order of its scatter-add (`Tensor.collate`) to adpy, so bitwise agreement with
 the reference's sums is not claimed.
 
+How each function is pinned (tests/test_oracle.py unless named):
+  primitive/conservative   round trip and p = rho R T
+  ghosts                   fixedValue/zeroGradient copies, slip (zero normal
+                           velocity), CBC_TOTAL_PT total-temperature and
+                           isentropic relations
+  grad_cell                exact for linear fields; zero for constants on
+                           an O-grid
+  second_order             exact face values for linear fields, both sides;
+                           limiter bounds and identity inside the bounds
+  euler_roe / euler_lf     consistency, conservation F(L,R,n) = -F(R,L,-n),
+                           rotational invariance, Roe upwinding of a contact,
+                           LF dissipation >= Roe
+  viscous_flux             zero for uniform fields; linear-T heat flux and
+                           Couette shear in closed form
+  accumulate / equation    sum_c V R = 0 on a periodic box; dtc by hand
+  step                     conservation, constant-source update, SSP-RK3
+                           third order, autograd vs central differences, and
+                           the shock tube against the exact solution
+                           (tests/golden/shocktube_exact_output.txt, the
+                           reference's cases/shockTube/output)
+  objective_value          drag and kinetic energy in closed form
+Parity unpinned: stabilise() (the semantics of adpy's `stabilise`, whose
+source is absent; only its use at riemann.py:52 is known) and the bitwise
+order of the reference's face->cell sums.
+
 Inputs: an oracle mesh (oracle.mesh_oracle.build) and a case spec:
   bcs[patch][field] = {'type': 'cyclic'|'zeroGradient'|'empty'|'inletOutlet'|
                        'symmetryPlane'|'slip'|'fixedValue'|'calculated'|

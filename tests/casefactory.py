@@ -109,7 +109,7 @@ def sod_state(case, split=0.5):
     return state_from_prims(U, T, p)
 
 
-def mixed_box(seed=0, cbc='CBC_UPT', riemann='roe', boundaryRiemann='roe', sutherland=False):
+def mixed_box(seed=0, cbc='CBC_UPT', riemann='roe', boundaryRiemann='roe', sutherland=False, objective=None):
     """Every ghost rule at once on a small random-renumbered box: fixedValue
     inlet, characteristic outlet, slip and no-slip walls, empty sides."""
     patches = {'x0': ('inlet', 'patch'), 'x1': ('outlet', 'patch'),
@@ -137,7 +137,8 @@ def mixed_box(seed=0, cbc='CBC_UPT', riemann='roe', boundaryRiemann='roe', suthe
     else:
         bcs['outlet'] = {'U': {'type': 'zeroGradient'}, 'T': {'type': 'calculated', 'value': np.array([299.])},
                          'p': {'type': 'calculated', 'value': np.array([1.0e5])}}
-    return CaseDef(md, bcs, 1.8e-5, riemann=riemann, boundaryRiemann=boundaryRiemann, sutherland=sutherland)
+    return CaseDef(md, bcs, 1.8e-5, riemann=riemann, boundaryRiemann=boundaryRiemann, sutherland=sutherland,
+                   objective=objective)
 
 
 def smooth_state(case, seed=0, U=(25., 3., 1.), amp=0.05):

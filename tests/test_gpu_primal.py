@@ -35,6 +35,10 @@ CASES = {
     'mixed_upt': lambda: (lambda c: (c, cf.smooth_state(c, 1)))(cf.mixed_box(1, 'CBC_UPT')),
     'mixed_totalpt_lf': lambda: (lambda c: (c, cf.smooth_state(c, 2)))(cf.mixed_box(2, 'CBC_TOTAL_PT', 'lf', 'lf', True)),
     'mixed_calc': lambda: (lambda c: (c, cf.smooth_state(c, 3)))(cf.mixed_box(3, 'calculated')),
+    # drag on the no-slip wall 'top' (T zeroGradient there: the ghost T
+    # follows the owner), Sutherland viscosity
+    'mixed_drag': lambda: (lambda c: (c, cf.smooth_state(c, 4)))(cf.mixed_box(4, 'CBC_UPT', sutherland=True,
+                                                                              objective=('drag', 'top'))),
 }
 
 
