@@ -79,6 +79,12 @@ SIGNATURES = {
     'adfvm_ctx_destroy': (None, [vp]),
     'adfvm_last_error': (C.c_char_p, [vp]),
     'adfvm_ctx_precision': (C.c_int, [vp]),
+    'adfvm_set_segments': (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64)]),
+    'adfvm_segments': (C.c_int, [vp]),
+    'adfvm_autotune': (C.c_int, [vp, vp, C.c_int32, vp]),
+    'adfvm_kernel_config': (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_double)]),
+    'adfvm_set_kernel_block': (C.c_int, [vp, C.c_char_p, C.c_int32]),
     'adfvm_primal_step': (C.c_int, [vp, vp, C.c_double, vp, vp, vp, vp, vp]),
     'adfvm_tape_bytes': (C.c_int64, [vp]),
     'adfvm_adjoint_step': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp, vp]),

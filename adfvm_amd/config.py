@@ -61,10 +61,15 @@ def parse_user_args(argv=None):
 
 def importModule(localsDict, module):
     """Case files are Python modules whose public globals become the app's
-    (adFVM/config.py:148-156)."""
+    (adFVM/config.py:148-156).  The file is executed afresh on every call, so
+    that several runs in one process each get their own solver objects."""
+    import importlib.util
     caseDir, caseFile = os.path.split(module)
     sys.path.append(os.path.abspath(caseDir))
-    mod = __import__(caseFile.split('.')[0])
+    path = module if module.endswith('.py') else module + '.py'
+    spec = importlib.util.spec_from_file_location(caseFile.split('.')[0], path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
     for attr in dir(mod):
         if not attr.startswith('_'):
             localsDict[attr] = getattr(mod, attr)

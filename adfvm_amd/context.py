@@ -173,8 +173,18 @@ class Context(object):
         k = self.tape_bytes() // self.precision.itemsize
         return self.torch.empty((n, k), dtype=self.tdtype, device='cuda')
 
+    def set_segments(self, cell_starts):
+        """Ensemble of meshes in this context: segment k owns cells
+        [cell_starts[k], cell_starts[k+1]); objectives then come per segment
+        (obj_out needs len(cell_starts)-1 doubles)."""
+        a = np.ascontiguousarray(cell_starts, np.int64)
+        self._chk(self.lib.adfvm_set_segments(self.handle, len(a) - 1, _lib.ptr(a, C.c_int64)))
+        self.nSeg = len(a) - 1
+
     def primal_step(self, u, dt, source=None, dtc_out=None, obj_out=None, tape=None, stream=None):
         self._check(u, (5, self.nIC))
+        if obj_out is not None:
+            assert obj_out.numel() >= getattr(self, 'nSeg', 1), 'obj_out needs one double per segment'
         if source is not None:
             self._check(source, (5, self.nIC))
         if tape is not None:
@@ -208,7 +218,9 @@ class Context(object):
         return out
 
     def objective(self, u, out=None, stream=None):
-        out = out if out is not None else self.torch.zeros(1, dtype=self.torch.float64, device='cuda')
+        n = getattr(self, 'nSeg', 1)
+        out = out if out is not None else self.torch.zeros(n, dtype=self.torch.float64, device='cuda')
+        assert out.numel() >= n
         self._chk(self.lib.adfvm_objective(self.handle, _dptr(u), _dptr(out), _stream(stream)))
         return out
 
@@ -256,6 +268,25 @@ class Context(object):
             self._chk(self.lib.adfvm_profile_entry(self.handle, i, buf, 64, C.byref(ms), C.byref(n)))
             out[buf.value.decode()] = (ms.value, n.value)
         return out
+
+    def autotune(self, u, reps=5, stream=None):
+        """Pick each stencil kernel's block size by timing it on state u;
+        returns {kernel: (block, ms per launch)}."""
+        self._check(u, (5, self.nIC))
+        self._chk(self.lib.adfvm_autotune(self.handle, _dptr(u), int(reps), _stream(stream)))
+        return self.kernel_config()
+
+    def kernel_config(self):
+        out = {}
+        for i in range(6):
+            buf = C.create_string_buffer(64)
+            b, ms = C.c_int32(), C.c_double()
+            self._chk(self.lib.adfvm_kernel_config(self.handle, i, buf, 64, C.byref(b), C.byref(ms)))
+            out[buf.value.decode()] = (b.value, ms.value)
+        return out
+
+    def set_kernel_block(self, name, block):
+        self._chk(self.lib.adfvm_set_kernel_block(self.handle, name.encode(), int(block)))
 
     def pack(self, rho, rhoU, rhoE, out=None, stream=None):
         out = out if out is not None else self.state()

@@ -28,6 +28,18 @@ using namespace adfvm;
 
 #define BLOCK 256
 
+// Workgroups of a launch are dealt to the 8 XCDs round-robin (block b runs
+// on XCD b % 8), and each XCD has its own L2.  The stencil kernels remap
+// block b to a logical block so that every XCD works through one contiguous
+// eighth of the cells (or faces): the neighbours a block gathers (RCM
+// order: within a few thousand rows) were then recently touched on the
+// same XCD and hit its L2.  The map is a bijection of [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return (int)(x * q + (x < r ? x : r) + k);
+}
+
 // ------------------------------------------------------------- tables ----
 // slot flags: bit 0 = this cell owns the face; bits 1-2 = slot kind
 enum { SLOT_INTERNAL = 0, SLOT_COUPLED = 1, SLOT_CHARACTERISTIC = 2, SLOT_BOUNDARY = 3 };
@@ -90,7 +102,8 @@ __device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict
 // record is a few 16-byte vector loads instead of one cache line per field:
 //   gradients        [nIC][GREC]   15 used
 //   face fluxes      [nF][FREC]    6 used (F*A, lam*A)
-//   face adjoints    [nF][2*AREC]  per side: gradient adjoint (15), primitive adjoint (5)
+//   face adjoints    [2][nF][AREC] owner sides, then neighbour sides: gradient adjoint (15),
+//                                  primitive adjoint (5)
 //   cell adjoints    [nIC][AREC]   same layout as one side of a face record
 constexpr int GREC = 16, FREC = 8, AREC = 20;
 
@@ -185,10 +198,10 @@ __global__ void __launch_bounds__(BLOCK) ghost_kernel(DMesh<T> m, Phys<T> ph, co
 }
 
 // Gauss cell gradients of U, T, p (op.gradCell, op.py:45-63).
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) grad_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+template <typename T, int BS>
+__global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                      const T* __restrict__ ghost, T* __restrict__ grad) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * BS + threadIdx.x;
     if (c >= m.nIC) return;
     const long nIC = m.nIC, nF = m.nF;
     T wc[5];
@@ -232,12 +245,12 @@ __global__ void __launch_bounds__(BLOCK) grad_rows_kernel(long nIC, const T* __r
 }
 
 // Block reduction of one double per thread into out[blockIdx.x]; op 0 sum, 1 max.
-template <int OP>
+template <int OP, int BS = BLOCK>
 __device__ void block_reduce_store(double v, double* out) {
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[BS];
     sh[threadIdx.x] = v;
     __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    for (int s = BS / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             double a = sh[threadIdx.x], b = sh[threadIdx.x + s];
             sh[threadIdx.x] = OP == 0 ? a + b : (a > b ? a : b);
@@ -274,8 +287,8 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(const double* __restric
 // Internal and coupled faces: RCF.flux (density.py:253-305) between the
 // owner and the neighbour (for a cyclic face, the paired internal cell);
 // characteristic / other boundary faces: the ghost as the right state.
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) face_flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+template <typename T, int BS>
+__global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                           const T* __restrict__ grad, const T* __restrict__ ghost,
                                                           T* __restrict__ fbuf) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -310,13 +323,13 @@ __global__ void __launch_bounds__(BLOCK) face_flux_kernel(DMesh<T> m, Phys<T> ph
 //   lam_c = (1/V_c) sum_slots (lam A)_f      (op.absDiv, op.py:22-29)
 // MODE 0: uout = a0*u0 + ai*ui + (-beta*(L-S))*dt   (timestep.update, timestep.py:35-45)
 // MODE 1: uout = L (the residual)
-template <typename T, int MODE>
-__global__ void __launch_bounds__(BLOCK) stage_kernel(DMesh<T> m, const T* __restrict__ fbuf,
+template <typename T, int MODE, int BS>
+__global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restrict__ fbuf,
                                                       const T* __restrict__ ui, const T* __restrict__ u0, T a0,
                                                       T ai, T nbeta, T dt, const T* __restrict__ S,
                                                       T* __restrict__ uout, T* __restrict__ dtcCell,
                                                       double* __restrict__ dtcPartial) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * BS + threadIdx.x;
     const long nIC = m.nIC, nF = m.nF;
     double dtcv = 0.0;
     if (c < m.nIC) {
@@ -347,7 +360,7 @@ __global__ void __launch_bounds__(BLOCK) stage_kernel(DMesh<T> m, const T* __res
         if (dtcCell) dtcCell[c] = lamSum;
         dtcv = (double)lamSum;
     }
-    if (dtcPartial) block_reduce_store<1>(dtcv, dtcPartial);
+    if (dtcPartial) block_reduce_store<1, BS>(dtcv, dtcPartial);
 }
 
 // Objective partial sums (FP64): kinetic energy 0.5*rho*|U|^2*V per cell.
@@ -363,6 +376,46 @@ __global__ void __launch_bounds__(BLOCK) ke_kernel(DMesh<T> m, const T* __restri
         v = (double)ke;
     }
     block_reduce_store<0>(v, partial);
+}
+
+// Kinetic energy per cell segment (an ensemble of meshes in one context):
+// block b covers cells segCell[k] + 256*(b - segBlock[k]) .. of segment
+// k = blockSeg[b], so each segment's partials group its cells exactly as a
+// context holding that segment alone would.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) ke_seg_kernel(DMesh<T> m, const T* __restrict__ u,
+                                                       const long* __restrict__ segCell,
+                                                       const int* __restrict__ segBlock,
+                                                       const int* __restrict__ blockSeg, double* partial) {
+    const int k = blockSeg[blockIdx.x];
+    const long c = segCell[k] + (long)(blockIdx.x - segBlock[k]) * BLOCK + threadIdx.x;
+    const long nIC = m.nIC;
+    double v = 0.0;
+    if (c < segCell[k + 1]) {
+        T rho = u[c];
+        T Ux = u[nIC + c] / rho, Uy = u[2 * nIC + c] / rho, Uz = u[3 * nIC + c] / rho;
+        T ke = T(0.5) * rho * (Ux * Ux + Uy * Uy + Uz * Uz) * m.vol[c];
+        v = (double)ke;
+    }
+    block_reduce_store<0>(v, partial);
+}
+
+// One block per segment: out[k] = sum of its partials, folded as
+// finalize_kernel folds a whole context's.
+__global__ void __launch_bounds__(BLOCK) finalize_seg_kernel(const double* __restrict__ partials,
+                                                             const int* __restrict__ segBlock, double* out) {
+    const int k = blockIdx.x;
+    const int b0 = segBlock[k], n = segBlock[k + 1] - b0;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) v = v + partials[b0 + i];
+    __shared__ double sh[BLOCK];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = BLOCK / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = sh[0];
 }
 
 // Drag on one patch (templates/cylinder_test.py:9-19): sum (p0 nx - mu(T0)(U0x-Uix)/delta) A.
@@ -564,10 +617,16 @@ struct adfvm_ctx {
     void* adjAcc = nullptr;    // [5][nIC] adjoint of u0 accumulated over the stages
     void* adjG = nullptr;      // [nIC][AREC] cell adjoint records (gradient 15, primitive 5)
     double* partial = nullptr; int nPartial = 0;
+    // block size of each tunable stencil kernel (adfvm_autotune) and its measured time
+    int bs[6] = {256, 256, 256, 256, 256, 256};
+    double tuneMs[6] = {-1, -1, -1, -1, -1, -1};
+    // cell segments (adfvm_set_segments): objective per segment
+    int nSeg = 1, nSegBlocks = 0;
+    long* segCell = nullptr; int* segBlock = nullptr; int* blockSeg = nullptr;
     double* scal = nullptr;    // small device scalars
     int* fOther = nullptr; unsigned char* fKind = nullptr;
     void* fbuf = nullptr;      // [6][nF] face fluxes * area
-    void* frec = nullptr;      // [40][nF] face adjoint records (owner side 20, neighbour side 20)
+    void* frec = nullptr;      // [2][nF][AREC] face adjoint records (owner sides, neighbour sides)
     // per-kernel event timing (adfvm_profile)
     bool prof = false;
     std::vector<hipEvent_t> evPool;
@@ -616,6 +675,23 @@ int set_err(adfvm_ctx* ctx, int code, const std::string& m) {
     } while (0)
 
 int grid_for(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
+int grid_bs(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// Tunable stencil kernels and the block sizes they are built for.
+enum { K_GRAD = 0, K_FLUX, K_STAGE, K_FVJP, K_GATHER, K_GVJP, K_TUNABLE };
+const char* const kTunableName[K_TUNABLE] = {"grad_kernel", "face_flux_kernel", "stage_kernel",
+                                              "face_vjp_kernel", "vjp_gather_kernel", "grad_vjp_kernel"};
+const int kBlockChoices[4] = {64, 128, 256, 512};
+
+// Calls f(std::integral_constant<int, BS>) for the block size bs.
+template <typename F> void with_block(int bs, F&& f) {
+    switch (bs) {
+        case 64: f(std::integral_constant<int, 64>()); break;
+        case 128: f(std::integral_constant<int, 128>()); break;
+        case 512: f(std::integral_constant<int, 512>()); break;
+        default: f(std::integral_constant<int, 256>()); break;
+    }
+}
 
 template <typename T> Phys<T> phys_cast(const Phys<double>& p) {
     Phys<T> o;
@@ -709,13 +785,20 @@ template <typename T> void launch_ghosts(adfvm_ctx* c, const T* u, hipStream_t s
 
 template <typename T> void launch_grad(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
     KTimer kt(c, "grad_kernel", s);
-    grad_kernel<T><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), grad);
+    with_block(c->bs[K_GRAD], [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        grad_kernel<T, BS><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost),
+                                                               grad);
+    });
 }
 
 template <typename T> void launch_face_flux(adfvm_ctx* c, const T* u, const T* grad, hipStream_t s) {
     KTimer kt(c, "face_flux_kernel", s);
-    face_flux_kernel<T><<<grid_for(c->nF), BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, grad, P<T>(c->ghost),
-                                                           P<T>(c->fbuf));
+    with_block(c->bs[K_FLUX], [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        face_flux_kernel<T, BS><<<grid_bs(c->nF, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, grad,
+                                                                   P<T>(c->ghost), P<T>(c->fbuf));
+    });
 }
 
 struct Stage { double a0, ai, beta; };   // u_{i+1} = a0*u0 + ai*u_i - beta*(L-S)*dt
@@ -724,14 +807,32 @@ template <typename T, int MODE>
 void launch_stage(adfvm_ctx* c, const T* ui, const T* u0, const Stage& st, double dt, const T* S, T* out,
                   T* dtcCell, double* dtcPartial, hipStream_t s) {
     KTimer kt(c, MODE == 0 ? "stage_kernel" : "residual_kernel", s);
-    stage_kernel<T, MODE><<<grid_for(c->nIC), BLOCK, 0, s>>>(dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.a0,
-                                                              (T)st.ai, (T)(-st.beta), (T)dt, S, out, dtcCell,
-                                                              dtcPartial);
+    with_block(c->bs[K_STAGE], [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        stage_kernel<T, MODE, BS><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.a0,
+                                                                      (T)st.ai, (T)(-st.beta), (T)dt, S, out,
+                                                                      dtcCell, dtcPartial);
+    });
+}
+
+// Fluxes of all faces at state ui (gradients grad) and the stage update or
+// residual.  (A fused cell-centric variant that evaluates every face from
+// both sides instead of storing face records measured 2.3x slower: the face
+// flux is ALU-heavy.)
+template <typename T, int MODE>
+void launch_flux_and_stage(adfvm_ctx* c, const T* ui, const T* grad, const T* u0, const Stage& st, double dt,
+                           const T* S, T* out, T* dtcCell, double* dtcPartial, hipStream_t s) {
+    launch_face_flux<T>(c, ui, grad, s);
+    launch_stage<T, MODE>(c, ui, u0, st, dt, S, out, dtcCell, dtcPartial, s);
 }
 
 template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* out, hipStream_t s) {
     KTimer kt(c, "objective", s);
-    if (c->objective == ADFVM_OBJ_KINETIC_ENERGY) {
+    if (c->objective == ADFVM_OBJ_KINETIC_ENERGY && c->nSeg > 1) {
+        ke_seg_kernel<T><<<c->nSegBlocks, BLOCK, 0, s>>>(dmesh<T>(c), u, c->segCell, c->segBlock, c->blockSeg,
+                                                         c->partial);
+        finalize_seg_kernel<<<c->nSeg, BLOCK, 0, s>>>(c->partial, c->segBlock, out);
+    } else if (c->objective == ADFVM_OBJ_KINETIC_ENERGY) {
         int g = grid_for(c->nIC);
         ke_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), u, c->partial);
         finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
@@ -741,7 +842,7 @@ template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* ou
         drag_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), Pp.start, Pp.n, c->partial);
         finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
     } else {
-        fill_kernel<<<1, 1, 0, s>>>(out, 1, 0.0);
+        fill_kernel<<<1, 1, 0, s>>>(out, c->nSeg, 0.0);
     }
 }
 
@@ -785,9 +886,8 @@ int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, do
         launch_grad<T>(c, ui, grads[i], s);
         const bool wantDtc = (i == objStage) && dtc_out;
         const T* u0 = (i > 0 && st[i].a0 != 0.0) ? (const T*)u : nullptr;
-        int g = grid_for(c->nIC);
-        launch_face_flux<T>(c, ui, grads[i], s);
-        launch_stage<T, 0>(c, ui, u0, st[i], dt, S, out, nullptr, wantDtc ? c->partial : nullptr, s);
+        int g = grid_bs(c->nIC, c->bs[K_STAGE]);
+        launch_flux_and_stage<T, 0>(c, ui, grads[i], u0, st[i], dt, S, out, nullptr, wantDtc ? c->partial : nullptr, s);
         if (wantDtc) finalize_kernel<1><<<1, BLOCK, 0, s>>>(c->partial, g, dtc_out);
         ui = out;
     }
@@ -918,7 +1018,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     rc = c->prec == 4 ? create_typed<float>(c, d) : create_typed<double>(c, d);
     if (rc != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->patches, c->hpatches)) != ADFVM_OK) return rc;
-    c->nPartial = std::max(grid_for(std::max<long>(nIC, nF)), 1);
+    c->nPartial = grid_bs(std::max<long>(nIC, nF), kBlockChoices[0]) + 1;   // the smallest block size
     HIPCHK(c, hipMalloc((void**)&c->partial, c->nPartial * sizeof(double)));
     HIPCHK(c, hipMalloc((void**)&c->scal, 16 * sizeof(double)));
     return ADFVM_OK;
@@ -929,7 +1029,8 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
     void* ptrs[] = {c->geo, c->vol, c->bcval, c->sFace, c->sOther, c->sFlags, c->sPair, c->owner, c->bPatch,
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
                     c->grad2, c->adjAcc, c->adjG,
-                    c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec};
+                    c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
+                    c->segCell, c->segBlock, c->blockSeg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -939,6 +1040,37 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
 
 extern "C" const char* adfvm_last_error(const adfvm_ctx* c) { return c ? c->err.c_str() : "null context"; }
 extern "C" int adfvm_ctx_precision(const adfvm_ctx* c) { return c ? c->prec : 0; }
+
+extern "C" int adfvm_set_segments(adfvm_ctx* c, int32_t nSeg, const int64_t* cellStart) {
+    if (!c || nSeg < 1 || !cellStart) return ADFVM_ERR_INVALID;
+    if (cellStart[0] != 0 || cellStart[nSeg] != c->nIC)
+        return set_err(c, ADFVM_ERR_INVALID, "segments must start at cell 0 and end at nInternalCells");
+    for (int k = 0; k < nSeg; k++)
+        if (cellStart[k + 1] <= cellStart[k]) return set_err(c, ADFVM_ERR_INVALID, "empty or decreasing segment");
+    if (nSeg > 1 && c->objective == ADFVM_OBJ_DRAG)
+        return set_err(c, ADFVM_ERR_UNSUPPORTED, "segments with the drag objective");
+    std::vector<long> cells(cellStart, cellStart + nSeg + 1);
+    std::vector<int> first(nSeg + 1), owner;
+    first[0] = 0;
+    for (int k = 0; k < nSeg; k++) {
+        int nb = grid_for(cells[k + 1] - cells[k]);
+        first[k + 1] = first[k] + nb;
+        for (int b = 0; b < nb; b++) owner.push_back(k);
+    }
+    if (first[nSeg] > c->nPartial) return set_err(c, ADFVM_ERR_INVALID, "too many segment blocks");
+    for (void* p : {(void*)c->segCell, (void*)c->segBlock, (void*)c->blockSeg})
+        if (p) (void)hipFree(p);
+    c->segCell = nullptr; c->segBlock = nullptr; c->blockSeg = nullptr;
+    int rc;
+    if ((rc = upload(c, (void**)&c->segCell, cells)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->segBlock, first)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->blockSeg, owner)) != ADFVM_OK) return rc;
+    c->nSeg = nSeg;
+    c->nSegBlocks = first[nSeg];
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_segments(const adfvm_ctx* c) { return c ? c->nSeg : 0; }
 
 #define DISPATCH(c, CALL_F, CALL_D) ((c)->prec == 4 ? (CALL_F) : (CALL_D))
 
@@ -961,8 +1093,7 @@ template <typename T> int gradients_t(adfvm_ctx* c, const T* u, T* grad, hipStre
 template <typename T> int residual_t(adfvm_ctx* c, const T* u, T* R, T* dtcCell, hipStream_t s) {
     launch_ghosts<T>(c, u, s);
     launch_grad<T>(c, u, P<T>(c->grad0), s);
-    launch_face_flux<T>(c, u, P<T>(c->grad0), s);
-    launch_stage<T, 1>(c, u, nullptr, Stage{0, 0, 0}, 0.0, nullptr, R, dtcCell, nullptr, s);
+    launch_flux_and_stage<T, 1>(c, u, P<T>(c->grad0), nullptr, Stage{0, 0, 0}, 0.0, nullptr, R, dtcCell, nullptr, s);
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
 }

@@ -194,6 +194,43 @@ def permute_and_rcm(mesh, seed=None, rcm=True):
     return mesh
 
 
+def concatenate(meshes, prefixes=None):
+    """Disjoint union of meshes as one mesh (an ensemble in one context):
+    mesh k's cells follow mesh k-1's; the internal faces of all meshes come
+    first, in mesh order, then every mesh's patches, renamed prefix + name
+    (cyclic partners renamed alike).  Each mesh's own cell and face order is
+    kept, so per-mesh geometry, slot tables and kernel arithmetic are those
+    of the mesh alone.  Returns (MeshData, cell offsets (len(meshes)+1,))."""
+    prefixes = prefixes or ['m%d_' % k for k in range(len(meshes))]
+    pts, fint, oint, nint, fb, ob = [], [], [], [], [], []
+    boundary = OrderedDict()
+    cur = sum(len(m.neighbour) for m in meshes)
+    p_off = c_off = 0
+    offsets = [0]
+    for m, pre in zip(meshes, prefixes):
+        nIF = len(m.neighbour)
+        pts.append(m.points)
+        fint.append(m.faces[:nIF] + p_off)
+        oint.append(m.owner[:nIF] + c_off)
+        nint.append(m.neighbour + c_off)
+        for name, b in sorted(m.boundary.items(), key=lambda kv: kv[1]['startFace']):
+            st, n = b['startFace'], b['nFaces']
+            fb.append(m.faces[st:st + n] + p_off)
+            ob.append(m.owner[st:st + n] + c_off)
+            nb = dict(b)
+            nb['startFace'] = cur
+            if 'neighbourPatch' in b:
+                nb['neighbourPatch'] = pre + b['neighbourPatch']
+            boundary[pre + name] = nb
+            cur += n
+        p_off += len(m.points)
+        c_off += m.nCells
+        offsets.append(c_off)
+    md = MeshData(np.concatenate(pts), np.concatenate(fint + fb), np.concatenate(oint + ob),
+                  np.concatenate(nint), boundary, c_off)
+    return md, np.array(offsets, np.int64)
+
+
 def box(n, lengths=(1., 1., 1.), origin=(0., 0., 0.), patches=None, x_coords=None):
     """Cartesian box of n=(nx,ny,nz) cells.  patches maps sides
     'x0','x1','y0','y1','z0','z1' -> (name, type[, neighbourPatch])."""

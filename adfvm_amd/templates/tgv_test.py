@@ -32,7 +32,9 @@ primal = RCF(caseDir,
 
 
 def perturb(fields, mesh, t):
-    mid = np.array([np.pi, np.pi, np.pi])
+    # centred where u = U0 (a stagnation point would leave only the
+    # quadratic response, and one-sided differences would not see the slope)
+    mid = np.array([np.pi / 2, np.pi, np.pi])
     r2 = ((mesh.cellCentres[:mesh.nInternalCells] - mid) ** 2).sum(1, keepdims=True)
     G = 1e-3 * np.exp(-r2 / 0.5)
     rho = G

@@ -68,6 +68,7 @@ def run_precision(case, u0, prec, K, W, dt, torch, profile, recompute=False):
     ctx = case.context(prec)
     nC = ctx.nIC
     u = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
+    tuned = ctx.autotune(u, reps=3)          # block size per stencil kernel (results do not depend on it)
     store = torch.empty((K, 5, nC), dtype=ctx.tdtype, device='cuda')
     tapes = None if recompute else ctx.new_tapes(K)
     ua = torch.zeros_like(u)
@@ -98,7 +99,7 @@ def run_precision(case, u0, prec, K, W, dt, torch, profile, recompute=False):
     torch.cuda.synchronize()
     t_p = time.perf_counter() - t0
     del ctx
-    return dict(t_pa=t_pa, t_p=t_p, prof=prof, finite=ok)
+    return dict(t_pa=t_pa, t_p=t_p, prof=prof, finite=ok, tuned={k: v[0] for k, v in tuned.items()})
 
 
 def l2_error(case, u0, steps, dt, torch):
@@ -143,6 +144,39 @@ def cpu_baseline(n=32, steps=8, dt=2e-4):
                                     '%.1f s' % (n, steps, t))
 
 
+def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
+    """Batched ensemble: n_instances Taylor-Green boxes, sizes drawn by seed
+    from {32, 48, 64, 96}^3, +-1% seeded velocity perturbation, advanced
+    primal + adjoint as one concatenated mesh, one HIP graph per checkpoint
+    segment, FP32 state, FP64 per-instance objectives."""
+    from adfvm_amd.ensemble import TaylorGreenEnsemble
+    t0 = time.time()
+    ens = TaylorGreenEnsemble(n_instances=n_instances, seed=seed)
+    print('[bench] ensemble of %d instances, %d cells, built in %.1fs' % (n_instances, ens.n_cells,
+                                                                           time.time() - t0),
+          file=sys.stderr, flush=True)
+    tuned = ens.ctx.autotune(ens.u0, reps=2)
+    ens.run(n_steps=interval, interval=interval)                  # warm-up (captures, runs one segment)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, ua = ens.run(n_steps=n_steps, interval=interval)
+    t = time.perf_counter() - t0
+    ok = bool(np.isfinite(J).all()) and bool(torch.isfinite(ua).all())
+    value = ens.n_cells * n_steps / t
+    return {'metric': 'cell-updates/s primal+adjoint, FP32 vs FP64, % HBM roofline, L2 err vs FP64',
+            'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': n_steps, 'warmup': 1,
+            'ms_per_step': t / n_steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': 'f32', 'data': 'synthetic (Taylor-Green initial fields, seeded +-1% velocity perturbation)',
+            'config': {'workload': 'batched ensemble: %d Taylor-Green instances, sizes %s^3 drawn by seed %d, '
+                                   'one concatenated mesh, checkpoints every %d steps, each segment one HIP '
+                                   'graph' % (n_instances, sorted(set(ens.sizes)), seed, interval),
+                       'n_cells': ens.n_cells, 'instances': n_instances,
+                       'size_counts': {str(k): ens.sizes.count(k) for k in sorted(set(ens.sizes))},
+                       'dt': ens.dt},
+            'block_sizes': {k: v[0] for k, v in tuned.items()}, 'finite': ok,
+            'objective_range': [float(J.min()), float(J.max())]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -153,6 +187,9 @@ def main():
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-recompute', action='store_true', help='skip the tape-less variant')
+    ap.add_argument('--ensemble', action='store_true',
+                    help='instead: the batched ensemble (64 Taylor-Green instances of 32^3..96^3, 200 steps '
+                         'primal+adjoint, checkpoints every 20, HIP-graph segments)')
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB five times (counter calibration)')
     args = ap.parse_args()
@@ -164,6 +201,9 @@ def main():
     from tests import casefactory as cf
     assert torch.cuda.is_available(), 'bench.py needs the GPU'
     torch.cuda.set_device(0)
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(torch)))
+        return
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device='cuda')
@@ -240,6 +280,7 @@ def main():
         'primal_cell_updates_per_s_fp32': nC * K / r32['t_p'],
         'value_fp32_recompute_stages': nC * K / r32r['t_pa'] if r32r else None,
         'kernel_ms_fp32': {k: round(v[0] / K, 4) for k, v in prof.items()},
+        'block_sizes_fp32': r32['tuned'],
         'finite': r32['finite'],
     }
     if r64 is not None:

@@ -184,6 +184,17 @@ void adfvm_ctx_destroy(adfvm_ctx* ctx);
 const char* adfvm_last_error(const adfvm_ctx* ctx);
 int adfvm_ctx_precision(const adfvm_ctx* ctx);
 
+/* Cell segments: an ensemble of independent meshes concatenated into one
+ * context, mesh k owning cells [cellStart[k], cellStart[k+1]).  The
+ * objective is then evaluated per segment: obj_out of adfvm_primal_step and
+ * adfvm_objective receive nSeg doubles, each summed exactly as a context
+ * holding that mesh alone would sum it.  cellStart: host, nSeg+1 entries
+ * from 0 to nInternalCells.  nSeg = 1 restores one objective.  No
+ * counterpart in the reference (one mesh per process there); it serves the
+ * batched ensemble of perturbed instances. */
+int adfvm_set_segments(adfvm_ctx* ctx, int32_t nSeg, const int64_t* cellStart);
+int adfvm_segments(const adfvm_ctx* ctx);
+
 /* One explicit time step in place: u <- step(u).  The reference's primal
  * Function (density.py:97-105: SSP-RK3 via timestep.py:27-55 of
  * RCF.equation, density.py:339-414).  source: device [5][nIC] or NULL.
@@ -210,6 +221,23 @@ int64_t adfvm_tape_bytes(const adfvm_ctx* ctx);
  * step's stages are recomputed from u_n. */
 int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* source,
                        void* ua, double obja, void* source_grad, const void* tape, void* stream);
+
+/* Block-size autotuner for the six stencil kernels (gradient, face flux,
+ * stage update, face adjoint, adjoint gather, gradient adjoint).  Times each
+ * at 64, 128, 256 and 512 threads per block (reps timed launches each) on
+ * the device state u and keeps the fastest for this context.  Results do not
+ * depend on the block size: each thread's arithmetic is unchanged and the
+ * only cross-thread combination in these kernels is a maximum.  Uses the
+ * context's work buffers and synchronizes the stream, so it is not for use
+ * inside graph capture.  No counterpart in the reference (adpy fixes its
+ * launch shapes at code generation). */
+int adfvm_autotune(adfvm_ctx* ctx, const void* u, int32_t reps, void* stream);
+/* i-th tunable kernel (0 <= i < 6): its name, the block size in use and the
+ * time per launch adfvm_autotune measured for it (-1 if not tuned). */
+int adfvm_kernel_config(const adfvm_ctx* ctx, int32_t i, char* name, int32_t len, int32_t* block,
+                        double* ms);
+/* Sets one tunable kernel's block size (64, 128, 256 or 512) by name. */
+int adfvm_set_kernel_block(adfvm_ctx* ctx, const char* name, int32_t block);
 
 /* Operator-level entry points (the hot path's pieces, used by tests). */
 
