@@ -17,8 +17,10 @@ LIB = os.path.join(HERE, 'libadfvm.so')
 ARCH = 'gfx950'
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
-# packed FP32 instructions are switched off for all device code
-HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=' + ARCH,
+# packed FP32 instructions are switched off for all device code; multiply-add
+# contraction is decided per source expression (-ffp-contract=on), not by the
+# backend, so results do not depend on launch bounds or instruction scheduling
+HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=' + ARCH, '-ffp-contract=on',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-munsafe-fp-atomics', '-I', INCLUDE, '-I', CSRC]
 CXX_FLAGS = ['-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', '-I', INCLUDE]

@@ -57,6 +57,8 @@ template <typename T> struct DMesh {
     int nIC, nIF, nF, nBF;
     const T* geo;           // [G_ROWS][nF]
     const T* vol;           // [nIC]
+    const T* sgeo;          // [6][4][nIC]: per cell slot, S*n signed outward for the cell (3) and
+                            // the weight of the other side's value in the face value (op.py:47-58)
     const int* sFace;       // [6][nIC]
     const int* sOther;      // [6][nIC]: internal cell, or -1-b for boundary face b
     const unsigned char* sFlags;  // [6][nIC]
@@ -64,6 +66,8 @@ template <typename T> struct DMesh {
     const int* owner;       // [nF]
     const int* fOther;      // [nF]: neighbour, paired internal cell (cyclic) or -1-b (ghost)
     const unsigned char* fKind;  // [nF]: FACE_INTERIOR / FACE_CHARACTERISTIC / FACE_BOUNDARY
+    const int* pos1;        // [nF]: slot of the face's neighbour-side adjoint record, -1 if none
+    const int* c1off;       // [nIC+1]: first neighbour-side record slot of each cell
     const int* bPatch;      // [nBF]
     const DPatch* patches;
     const T* bcval;
@@ -102,41 +106,27 @@ __device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict
 // record is a few 16-byte vector loads instead of one cache line per field:
 //   gradients        [nIC][GREC]   15 used
 //   face fluxes      [nF][FREC]    6 used (F*A, lam*A)
-//   face adjoints    [2][nF][AREC] owner sides, then neighbour sides: gradient adjoint (15),
-//                                  primitive adjoint (5)
+//   face adjoints    [2][nF][AREC] owner sides by face, then neighbour sides in gather
+//                                  order: gradient adjoint (15), primitive adjoint (5)
 //   cell adjoints    [nIC][AREC]   same layout as one side of a face record
 constexpr int GREC = 16, FREC = 8, AREC = 20;
 
-template <typename T> struct Vec16 { using type = typename std::conditional<sizeof(T) == 4, float4, double2>::type; };
-
-// N contiguous values from a 16-byte aligned address (N a multiple of 16/sizeof(T))
+// N contiguous values from / to a 16-byte aligned address (N a multiple of
+// 16/sizeof(T)), as 16-byte memory operations.  The copies go through
+// __builtin_memcpy so that no object is read through a pointer of another
+// type: the T values are never accessed as a vector type or vice versa.
 template <int N, typename T> __device__ __forceinline__ void vload(const T* __restrict__ p, T* out) {
-    using V = typename Vec16<T>::type;
-    constexpr int W = 16 / sizeof(T);
-    static_assert(N % W == 0, "vector width");
-    const V* q = reinterpret_cast<const V*>(p);
+    static_assert((N * sizeof(T)) % 16 == 0, "vector width");
+    const T* a = static_cast<const T*>(__builtin_assume_aligned(p, 16));
 #pragma unroll
-    for (int k = 0; k < N / W; k++) {
-        V v = q[k];
-        const T* s = reinterpret_cast<const T*>(&v);
-#pragma unroll
-        for (int i = 0; i < W; i++) out[k * W + i] = s[i];
-    }
+    for (int k = 0; k < (int)(N * sizeof(T) / 16); k++) __builtin_memcpy(out + k * (16 / sizeof(T)), a + k * (16 / sizeof(T)), 16);
 }
 
 template <int N, typename T> __device__ __forceinline__ void vstore(T* __restrict__ p, const T* in) {
-    using V = typename Vec16<T>::type;
-    constexpr int W = 16 / sizeof(T);
-    static_assert(N % W == 0, "vector width");
-    V* q = reinterpret_cast<V*>(p);
+    static_assert((N * sizeof(T)) % 16 == 0, "vector width");
+    T* a = static_cast<T*>(__builtin_assume_aligned(p, 16));
 #pragma unroll
-    for (int k = 0; k < N / W; k++) {
-        V v;
-        T* d = reinterpret_cast<T*>(&v);
-#pragma unroll
-        for (int i = 0; i < W; i++) d[i] = in[k * W + i];
-        q[k] = v;
-    }
+    for (int k = 0; k < (int)(N * sizeof(T) / 16); k++) __builtin_memcpy(a + k * (16 / sizeof(T)), in + k * (16 / sizeof(T)), 16);
 }
 
 // the 15 gradient entries of cell c (g has room for GREC values)
@@ -203,23 +193,18 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
                                                      const T* __restrict__ ghost, T* __restrict__ grad) {
     int c = xcd_block() * BS + threadIdx.x;
     if (c >= m.nIC) return;
-    const long nIC = m.nIC, nF = m.nF;
+    const long nIC = m.nIC;
     T wc[5];
     load_prim(ph, u, nIC, ghost, m.nBF, c, wc);
     T g[15];
     for (int i = 0; i < 15; i++) g[i] = T(0);
     for (int j = 0; j < 6; j++) {
-        const int f = m.sFace[j * nIC + c];
         const int o = m.sOther[j * nIC + c];
-        const bool own = m.sFlags[j * nIC + c] & 1;
         T wo[5];
         load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
-        T S = m.geo[G_A * nF + f];
-        T N[3] = {m.geo[G_NX * nF + f], m.geo[G_NY * nF + f], m.geo[G_NZ * nF + f]};
-        T w = m.geo[G_W * nF + f];
-        if (!own) { N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2]; }
-        else w = T(1) - w;
-        T SN[3] = {S * N[0], S * N[1], S * N[2]};
+        const T* sg = m.sgeo + (long)(4 * j) * nIC + c;
+        const T SN[3] = {sg[0], sg[nIC], sg[2 * nIC]};
+        const T w = sg[3 * nIC];
         for (int i = 0; i < 5; i++) {
             T phiF = wc[i] * (T(1) - w) + wo[i] * w;
             g[3 * i + 0] += phiF * SN[0];
@@ -612,6 +597,7 @@ struct adfvm_ctx {
     int* cellNbFull = nullptr;   // reference cellNeighbours [6][nIC]
     int* nbFull = nullptr;       // ghost-extended neighbour [nF]
     void* ghost = nullptr;
+    void* sgeo = nullptr;      // [6][4][nIC] cell-slot gradient geometry
     void* u1 = nullptr; void* u2 = nullptr;
     void* grad0 = nullptr; void* grad1 = nullptr; void* grad2 = nullptr;
     void* adjAcc = nullptr;    // [5][nIC] adjoint of u0 accumulated over the stages
@@ -625,8 +611,9 @@ struct adfvm_ctx {
     long* segCell = nullptr; int* segBlock = nullptr; int* blockSeg = nullptr;
     double* scal = nullptr;    // small device scalars
     int* fOther = nullptr; unsigned char* fKind = nullptr;
+    int* pos1 = nullptr; int* c1off = nullptr;
     void* fbuf = nullptr;      // [6][nF] face fluxes * area
-    void* frec = nullptr;      // [2][nF][AREC] face adjoint records (owner sides, neighbour sides)
+    void* frec = nullptr;      // [2][nF][AREC] face adjoint records (owner sides, neighbour sides in gather order)
     // per-kernel event timing (adfvm_profile)
     bool prof = false;
     std::vector<hipEvent_t> evPool;
@@ -704,10 +691,11 @@ template <typename T> Phys<T> phys_cast(const Phys<double>& p) {
 template <typename T> DMesh<T> dmesh(const adfvm_ctx* c) {
     DMesh<T> m;
     m.nIC = c->nIC; m.nIF = c->nIF; m.nF = c->nF; m.nBF = c->nBF;
-    m.geo = (const T*)c->geo; m.vol = (const T*)c->vol;
+    m.geo = (const T*)c->geo; m.vol = (const T*)c->vol; m.sgeo = (const T*)c->sgeo;
     m.sFace = c->sFace; m.sOther = c->sOther; m.sFlags = c->sFlags; m.sPair = c->sPair;
     m.owner = c->owner; m.bPatch = c->bPatch; m.patches = c->patches; m.bcval = (const T*)c->bcval;
     m.fOther = c->fOther; m.fKind = c->fKind;
+    m.pos1 = c->pos1; m.c1off = c->c1off;
     return m;
 }
 
@@ -738,6 +726,21 @@ template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* 
     std::vector<T> vol(nIC);
     for (long c = 0; c < nIC; c++) vol[c] = (T)d->volumes[c];
     if ((rc = upload(ctx, &ctx->vol, vol)) != ADFVM_OK) return rc;
+    // Gauss-gradient geometry per cell slot, computed in FP64 and cast once:
+    // the cell's outward S*n and the weight w' of the other side's value
+    // (phi_f = phi_c (1 - w') + phi_other w'), streamed by the gradient
+    // kernels instead of gathering face rows.
+    std::vector<T> sg((size_t)24 * nIC);
+    for (long c = 0; c < nIC; c++)
+        for (int j = 0; j < 6; j++) {
+            const long f = d->cellFaces[6 * c + j];
+            const bool own = d->cellOwner[6 * c + j] != 0;
+            const double sgn = own ? 1.0 : -1.0;
+            const double w = d->weights[f];
+            for (int k = 0; k < 3; k++) sg[(size_t)(4 * j + k) * nIC + c] = (T)(sgn * (d->areas[f] * d->normals[3 * f + k]));
+            sg[(size_t)(4 * j + 3) * nIC + c] = (T)(own ? 1.0 - w : w);
+        }
+    if ((rc = upload(ctx, &ctx->sgeo, sg)) != ADFVM_OK) return rc;
     // boundary-condition values
     std::vector<T> vals;
     for (int p = 0; p < d->nPatches; p++) {
@@ -1003,7 +1006,33 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
             fKind[f] = Pd.fluxKind == ADFVM_FLUX_CHARACTERISTIC ? FACE_CHARACTERISTIC : FACE_BOUNDARY;
         }
     }
+    // Neighbour-side face adjoint records are stored in the order the cells
+    // gather them: cell cc's records, in its slot order, at c1off[cc] ...
+    // (a face's neighbour side is needed by exactly one cell: the internal
+    // neighbour, or the partner of a cyclic face).
+    std::vector<int> pos1(nF, -1), c1off(nIC + 1, 0);
+    {
+        int k = 0;
+        for (long cc = 0; cc < nIC; cc++) {
+            c1off[cc] = k;
+            for (int j = 0; j < 6; j++) {
+                const int fl = sFlags[j * nIC + cc];
+                const int kind = fl >> 1;
+                int g = -1;
+                if (kind == SLOT_INTERNAL && !(fl & 1)) g = sFace[j * nIC + cc];
+                else if (kind == SLOT_COUPLED) g = sPair[j * nIC + cc];
+                if (g < 0) continue;
+                if (pos1[g] != -1) return bad("face with two neighbour-side slots");
+                pos1[g] = k++;
+            }
+        }
+        c1off[nIC] = k;
+        for (long f = 0; f < nF; f++)
+            if (fKind[f] == FACE_INTERIOR && pos1[f] < 0) return bad("face without a neighbour-side slot");
+    }
     int rc;
+    if ((rc = upload(c, (void**)&c->pos1, pos1)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->c1off, c1off)) != ADFVM_OK) return rc;
     std::vector<int> owner(d->owner, d->owner + nF), nbFull(d->neighbour, d->neighbour + nF);
     if ((rc = upload(c, (void**)&c->fOther, fOther)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->fKind, fKind)) != ADFVM_OK) return rc;
@@ -1030,7 +1059,7 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
-                    c->segCell, c->segBlock, c->blockSeg};
+                    c->segCell, c->segBlock, c->blockSeg, c->pos1, c->c1off, c->sgeo};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }

@@ -121,7 +121,7 @@ def l2_error(case, u0, steps, dt, torch):
     return err
 
 
-def cpu_baseline(n=32, steps=8, dt=2e-4):
+def cpu_baseline(n=48, steps=20, dt=2e-4):
     """The oracle (torch float64 on the host, autograd reverse) on a bounded
     sample: TGV n^3, `steps` steps forward + reverse."""
     import torch
