@@ -35,15 +35,18 @@ class DeviceStepper(object):
     """The checkpoint.stepper interface over a Context.
 
     dts[n] is step n's dt; obja(n) its objective weight; ua the adjoint
-    state (updated in place); source_grad, if given, accumulates the source
-    gradient; after_reverse(n, u_n) runs after step n is reversed."""
+    state (updated in place); source_grad and bc_grad, if given, accumulate
+    the source and boundary-value gradients; after_reverse(n, u_n) runs after
+    step n is reversed."""
 
-    def __init__(self, ctx, dts, ua, source=None, source_grad=None, obja=1.0, tapes=True, after_reverse=None):
+    def __init__(self, ctx, dts, ua, source=None, source_grad=None, obja=1.0, tapes=True, after_reverse=None,
+                 bc_grad=None):
         self.ctx = ctx
         self.dts = dts
         self.ua = ua
         self.source = source
         self.source_grad = source_grad
+        self.bc_grad = bc_grad
         self.obja = obja if callable(obja) else (lambda n, w=float(obja): w)
         self.tapes = tapes
         self.after_reverse = after_reverse
@@ -60,25 +63,26 @@ class DeviceStepper(object):
 
     def reverse(self, u_n, n, tape):
         self.ctx.adjoint_step(u_n, self.dts[n], self.ua, obja=self.obja(n), source=self.source,
-                              source_grad=self.source_grad, tape=tape)
+                              source_grad=self.source_grad, bc_grad=self.bc_grad, tape=tape)
         if self.after_reverse is not None:
             self.after_reverse(n, u_n)
 
 
 def sweep(ctx, u0, dts, schedule='store_all', interval=None, snapshots=None, obja=1.0, ua=None, source=None,
-          source_grad=None, tapes=True, after_reverse=None, checkpoints=None):
+          source_grad=None, tapes=True, after_reverse=None, checkpoints=None, bc_grad=None):
     """Reverse nSteps = len(dts) steps from the device state u0 (unchanged).
 
     Returns ua = d/du0 of sum_n obja(n) J_n + <ua_in, u_N> (ua_in = the given
     ua, zero by default), the objective J_n evaluated inside step n as the
-    primal step does.  source_grad (device (5, nIC)) accumulates the source
-    gradient of the same functional.  schedule: 'store_all', 'uniform'
+    primal step does.  source_grad (device (5, nIC)) and bc_grad (device
+    (5, nBoundaryFaces), Context.bc_state) accumulate the source and
+    boundary-value gradients of the same functional.  schedule: 'store_all', 'uniform'
     (interval steps per segment; `checkpoints` may hold segment starts
     n -> state) or 'revolve' (`snapshots` extra states)."""
     n = len(dts)
     if ua is None:
         ua = ctx.state(0.)
-    st = DeviceStepper(ctx, dts, ua, source, source_grad, obja, tapes, after_reverse)
+    st = DeviceStepper(ctx, dts, ua, source, source_grad, obja, tapes, after_reverse, bc_grad)
     if schedule == 'store_all':
         checkpoint.store_all(st, u0, 0, n)
     elif schedule == 'uniform':
@@ -182,11 +186,13 @@ class Adjoint(object):
     def run(self, checkpointData, primalData, nSteps, writeInterval, perturb=(), parameters=('source',),
             source=None, sampleInterval=1, reportInterval=1, avgStart=0, runCheckpoints=2 ** 30, tapes=True,
             verbose=False):
-        """The reverse sweep of apps/adjoint.py:176-404 for the 'source'
-        parameter; returns the (unaveraged) sensitivities, one per
-        perturbation."""
-        if parameters and list(parameters)[0] != 'source':
-            raise NotImplementedError('parameter %r (only the source parameter is supported)' % (parameters[0],))
+        """The reverse sweep of apps/adjoint.py:176-404 for the parameter
+        parameters[0]: 'source' or a boundary-condition value
+        ('BCs', field, patch, key) (apps/adjoint.py:104-118); returns the
+        (unaveraged) sensitivities, one per perturbation."""
+        param = list(parameters)[0] if parameters else 'source'
+        if not (param == 'source' or (isinstance(param, tuple) and len(param) == 4 and param[0] == 'BCs')):
+            raise NotImplementedError('parameter %r (source and BC values are supported)' % (param,))
         primal = self.primal
         ctx = primal.ctx
         torch = primal._torch()
@@ -202,12 +208,22 @@ class Adjoint(object):
         ua = self.ua
 
         perturbations = []
+        bgrad = bc_rows = bc_faces = None
+        if param != 'source':
+            _, phi, patchID, key = param
+            bc_rows, bc_faces = ctx.bc_param(phi, patchID, key)
+            bgrad = ctx.bc_state()
         for p in perturb:
             values = p(None, mesh, 0)
             values = list(values) if isinstance(values, (list, tuple)) else [values]
-            vals = [np.asarray(v, np.float64).reshape(mesh.nInternalCells, -1) for v in values]
-            d = torch.as_tensor(np.concatenate([v.T for v in vals]), dtype=ctx.tdtype).to('cuda').contiguous()
-            perturbations.append(d)
+            if bgrad is None:
+                vals = [np.asarray(v, np.float64).reshape(mesh.nInternalCells, -1) for v in values]
+                d = np.concatenate([v.T for v in vals])
+            else:                        # one (nFaces, components) array, broadcast
+                k = bc_rows.stop - bc_rows.start
+                nf = bc_faces.stop - bc_faces.start
+                d = np.broadcast_to(np.asarray(values[0], np.float64).reshape(-1, k), (nf, k)).T
+            perturbations.append(torch.as_tensor(np.ascontiguousarray(d), dtype=ctx.tdtype).to('cuda').contiguous())
         sgrad = ctx.state(0.)
         dsum = torch.zeros(1, dtype=torch.float64, device='cuda')
         energy = torch.zeros(1, dtype=torch.float64, device='cuda')
@@ -241,17 +257,20 @@ class Adjoint(object):
             for r in range(1, writeInterval + 1):              # reverse count, apps/adjoint.py:244-249
                 k = writeInterval - r
                 ctx.adjoint_step(states[k], dts[k], ua, obja=obja, source=primal.source, source_grad=sgrad,
-                                 tape=tps[k])
+                                 bc_grad=bgrad, tape=tps[k])
                 if r % reportInterval == 0:
                     ctx.adjoint_energy(ua, states[k], out=energy)
                     energyTimeSeries.append(energy.item() ** 0.5)
                 states[k] = tps[k] = None
                 if r % sampleInterval == 0:
                     sens = []
+                    g = sgrad if bgrad is None else bgrad[bc_rows, bc_faces].contiguous()
                     for d in perturbations:
-                        ctx.dot(sgrad, d, out=dsum)
+                        ctx.dot(g, d, out=dsum)
                         sens.append(dsum.item())
                     sgrad.zero_()
+                    if bgrad is not None:
+                        bgrad.zero_()
                     if (nSteps - (primalIndex + k)) > avgStart:
                         for i, sv in enumerate(sens):
                             result[i] += sv

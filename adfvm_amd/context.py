@@ -62,6 +62,7 @@ class Context(object):
         self.nBF = mesh.nFaces - mesh.nInternalFaces
         self._keep = []
         names = list(mesh.boundary.keys())
+        self._bcinfo = {}
         descs = (_lib.PatchDesc * max(1, len(names)))()
         for i, name in enumerate(names):
             b = mesh.boundary[name]
@@ -86,6 +87,9 @@ class Context(object):
             d.neighbourPatch = names.index(b['neighbourPatch']) if flux == _lib.FLUX_COUPLED else -1
             d.bcU, d.bcT, d.bcp = [BC_CODES[fspec[f]['type']] for f in ('U', 'T', 'p')]
             d.cbc = cbc
+            self._bcinfo[name] = dict(index=i, cbc=cbc, types={f: fspec[f]['type'] for f in ('U', 'T', 'p')},
+                                      start=b['startFace'] - mesh.nInternalFaces, n=n,
+                                      direction=cbc == _lib.CBC_TOTAL_PT and fspec['p'].get('direction') is not None)
 
             def arr(v, k):
                 if v is None:
@@ -192,13 +196,73 @@ class Context(object):
         self._chk(self.lib.adfvm_primal_step(self.handle, _dptr(u), float(dt), _dptr(source), _dptr(dtc_out),
                                              _dptr(obj_out), _dptr(tape), _stream(stream)))
 
-    def adjoint_step(self, u_n, dt, ua, obja=1.0, source=None, source_grad=None, tape=None, stream=None):
+    def adjoint_step(self, u_n, dt, ua, obja=1.0, source=None, source_grad=None, bc_grad=None, tape=None,
+                     stream=None):
+        """bc_grad (device (5, nBoundaryFaces), from bc_state()): the
+        gradient with respect to the boundary-condition values is added to
+        it; bc_param() locates one ('BCs', field, patch, key) parameter in it."""
         self._check(u_n, (5, self.nIC))
         self._check(ua, (5, self.nIC))
+        for a in (source, source_grad):
+            if a is not None:
+                self._check(a, (5, self.nIC))
+        if bc_grad is not None:
+            self._check(bc_grad, (5, self.nBF))
         if tape is not None:
             assert tape.is_contiguous() and tape.numel() * tape.element_size() >= self.tape_bytes()
         self._chk(self.lib.adfvm_adjoint_step(self.handle, _dptr(u_n), float(dt), _dptr(source), _dptr(ua),
-                                              float(obja), _dptr(source_grad), _dptr(tape), _stream(stream)))
+                                              float(obja), _dptr(source_grad), _dptr(bc_grad), _dptr(tape),
+                                              _stream(stream)))
+
+    def bc_state(self):
+        """Zeroed (5, nBoundaryFaces) device array for adjoint_step's bc_grad."""
+        return self.torch.zeros((5, self.nBF), dtype=self.tdtype, device='cuda')
+
+    def _bc_slot(self, field, patch, key):
+        """(patch index, ADFVM_BCV_* slot, components) of the value array
+        ('BCs', field, patch, key): 'value' of a fixedValue / calculated
+        U, T or p; U0, T0, p0 of a CBC_UPT and Tt, pt, direction (when
+        given) of a CBC_TOTAL_PT, both on p (BCs.py:111-184)."""
+        if patch not in self._bcinfo:
+            raise KeyError('no patch %r' % patch)
+        info = self._bcinfo[patch]
+        slot = None
+        if info['cbc'] == _lib.CBC_UPT:
+            if field == 'p':
+                slot = {'U0': _lib.BCV_U, 'T0': _lib.BCV_T, 'p0': _lib.BCV_P}.get(key)
+        elif info['cbc'] == _lib.CBC_TOTAL_PT:
+            if field == 'p':
+                slot = {'Tt': _lib.BCV_TT, 'pt': _lib.BCV_PT}.get(key)
+                if key == 'direction' and info['direction']:
+                    slot = _lib.BCV_DIRECTION
+        elif info['types'].get(field) in ('fixedValue', 'calculated') and key == 'value':
+            slot = {'U': _lib.BCV_U, 'T': _lib.BCV_T, 'p': _lib.BCV_P}[field]
+        if slot is None:
+            raise ValueError('(%r, %r, %r) is not a boundary-condition value of this case' % (field, patch, key))
+        return info['index'], slot, 3 if slot in (_lib.BCV_U, _lib.BCV_DIRECTION) else 1
+
+    def bc_param(self, field, patch, key):
+        """(rows, faces) slices of a bc_grad array that hold the parameter
+        ('BCs', field, patch, key) of apps/adjoint.py:111-118: the gradient
+        with respect to that value array is bc_grad[rows, faces].T, shaped
+        (nFaces, components) like the value.  A 'calculated' value sets no
+        ghost after the first step and has no gradient rows."""
+        _, slot, _ = self._bc_slot(field, patch, key)
+        info = self._bcinfo[patch]
+        if info['cbc'] == _lib.CBC_NONE and info['types'][field] != 'fixedValue':
+            raise ValueError('(%r, %r, %r) is not a fixedValue' % (field, patch, key))
+        rows = {_lib.BCV_U: slice(0, 3), _lib.BCV_T: slice(3, 4), _lib.BCV_P: slice(4, 5), _lib.BCV_TT: slice(3, 4),
+                _lib.BCV_PT: slice(4, 5), _lib.BCV_DIRECTION: slice(0, 3)}[slot]
+        return rows, slice(info['start'], info['start'] + info['n'])
+
+    def set_bc_value(self, field, patch, key, value, stream=None):
+        """Replace the value array ('BCs', field, patch, key) on the device
+        (value: (nFaces, components) or broadcastable to it)."""
+        index, slot, k = self._bc_slot(field, patch, key)
+        n = self._bcinfo[patch]['n']
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(value, np.float64).reshape(-1, k), (n, k)))
+        self._chk(self.lib.adfvm_set_bc_values(self.handle, index, slot, C.c_void_p(a.ctypes.data), a.size,
+                                               _stream(stream)))
 
     def gradients(self, u, out=None, stream=None):
         self._check(u, (5, self.nIC))

@@ -1070,6 +1070,37 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
 extern "C" const char* adfvm_last_error(const adfvm_ctx* c) { return c ? c->err.c_str() : "null context"; }
 extern "C" int adfvm_ctx_precision(const adfvm_ctx* c) { return c ? c->prec : 0; }
 
+namespace {
+template <typename T> int set_bc_values_t(adfvm_ctx* c, long off, const double* v, long n, hipStream_t s) {
+    std::vector<T> h(v, v + n);
+    HIPCHK(c, hipMemcpyAsync((T*)c->bcval + off, h.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return ADFVM_OK;
+}
+}  // namespace
+
+extern "C" int adfvm_set_bc_values(adfvm_ctx* c, int32_t patch, int32_t slot, const double* values, int64_t n,
+                                   void* stream) {
+    if (!c || !values) return ADFVM_ERR_INVALID;
+    if (patch < 0 || patch >= c->nPatches) return set_err(c, ADFVM_ERR_INVALID, "patch index out of range");
+    const DPatch& D = c->hpatches[patch];
+    long off;
+    int k = 1;
+    switch (slot) {
+        case ADFVM_BCV_U: off = D.valU; k = 3; break;
+        case ADFVM_BCV_T: off = D.valT; break;
+        case ADFVM_BCV_P: off = D.valp; break;
+        case ADFVM_BCV_TT: off = D.valTt; break;
+        case ADFVM_BCV_PT: off = D.valpt; break;
+        case ADFVM_BCV_DIRECTION: off = D.valDir; k = 3; break;
+        default: return set_err(c, ADFVM_ERR_INVALID, "unknown boundary-value slot");
+    }
+    if (off < 0) return set_err(c, ADFVM_ERR_INVALID, "this patch was created without that value array");
+    if (n != (int64_t)k * D.n) return set_err(c, ADFVM_ERR_INVALID, "value count is not nFaces x components");
+    hipStream_t s = (hipStream_t)stream;
+    return c->prec == 4 ? set_bc_values_t<float>(c, off, values, n, s) : set_bc_values_t<double>(c, off, values, n, s);
+}
+
 extern "C" int adfvm_set_segments(adfvm_ctx* c, int32_t nSeg, const int64_t* cellStart) {
     if (!c || nSeg < 1 || !cellStart) return ADFVM_ERR_INVALID;
     if (cellStart[0] != 0 || cellStart[nSeg] != c->nIC)

@@ -91,8 +91,8 @@ class Solver(object):
         each step (and, with `tapes`, fills tapes[n] with step n's tape).
         source(fields, mesh, t) -> (rho, rhoU, rhoE) source terms;
         perturbation = (parameters, perturb) adds perturb(...) to the
-        'source' parameter for the run and reverts it after
-        (solver.py:243-277,406-407).
+        'source' parameter or a BC value ('BCs', field, patch, key) for the
+        run and reverts it after (solver.py:243-277,406-407).
         Returns the time-integrated objective (FP64), or the states."""
         torch = self._torch()
         if self.mesh.caseDir and os.path.isdir(foam.time_dir(self.mesh.caseDir, startTime)):
@@ -171,11 +171,25 @@ class Solver(object):
         values = perturb(self.getFields(), self.mesh, t)
         if not isinstance(values, list) or (len(parameters) == 1 and len(values) > 1):
             values = [values]
+        sign = -1. if revert else 1.
         for param, value in zip(parameters, values):
-            if param != 'source':
-                raise NotImplementedError('perturbation of %r (only the source parameter is supported)' % (param,))
-            sign = -1. if revert else 1.
-            self.updateSource([sign * np.asarray(v, np.float64) for v in value], perturb=True)
+            if param == 'source':
+                self.updateSource([sign * np.asarray(v, np.float64) for v in value], perturb=True)
+            elif isinstance(param, tuple) and param[0] == 'BCs':
+                # patch.inputs[k][1][:] += value (solver.py:262-270), on the
+                # host spec and the device copy
+                _, phi, patchID, key = param
+                self._perturb_bc(phi, patchID, key, sign * np.asarray(value, np.float64))
+            else:
+                raise NotImplementedError('perturbation of %r (source and BC values are supported)' % (param,))
+
+    def _perturb_bc(self, phi, patchID, key, delta):
+        from . import BCs
+        spec = self.boundaries[phi][patchID]
+        n = self.mesh.boundary[patchID]['nFaces']
+        k = 3 if key in ('U0', 'direction') or (key == 'value' and phi == 'U') else 1
+        spec[key] = BCs.parse_value(spec[key], n, k) + np.broadcast_to(delta.reshape(-1, k), (n, k))
+        self.ctx.set_bc_value(phi, patchID, key, spec[key])
 
     def _torch(self):
         import torch

@@ -111,6 +111,14 @@ const char* adfvm_mesh_last_error(void);
 #define ADFVM_CBC_UPT 1
 #define ADFVM_CBC_TOTAL_PT 2
 
+/* boundary-value arrays of a patch (the value pointers of adfvm_patch_desc) */
+#define ADFVM_BCV_U 0          /* valueU */
+#define ADFVM_BCV_T 1          /* valueT */
+#define ADFVM_BCV_P 2          /* valuep */
+#define ADFVM_BCV_TT 3         /* Tt */
+#define ADFVM_BCV_PT 4         /* pt */
+#define ADFVM_BCV_DIRECTION 5  /* direction */
+
 typedef struct {
     int32_t startFace, nFaces;
     int32_t fluxKind;        /* ADFVM_FLUX_* */
@@ -193,6 +201,17 @@ int adfvm_ctx_precision(const adfvm_ctx* ctx);
  * counterpart in the reference (one mesh per process there); it serves the
  * batched ensemble of perturbed instances. */
 int adfvm_set_segments(adfvm_ctx* ctx, int32_t nSeg, const int64_t* cellStart);
+
+/* Replaces one boundary-value array of patch `patch` (index into the
+ * creation-time patch array) on the device: slot ADFVM_BCV_*, values host
+ * FP64, n = nFaces x components of that array (3 for U and direction, else
+ * 1).  The array must have been given at creation.  Ordered on `stream`
+ * (returns after the copy).  Replaces the reference's in-place update of a
+ * patch's input array, patch.inputs[k][1][:] += value (BCs.py:45-57,
+ * Solver.run's doPerturb solver.py:262-270): there those arrays are passed
+ * with every Function call, here they stay on the device. */
+int adfvm_set_bc_values(adfvm_ctx* ctx, int32_t patch, int32_t slot, const double* values, int64_t n,
+                        void* stream);
 int adfvm_segments(const adfvm_ctx* ctx);
 
 /* One explicit time step in place: u <- step(u).  The reference's primal
@@ -217,10 +236,19 @@ int64_t adfvm_tape_bytes(const adfvm_ctx* ctx);
  * the step's output, out: adjoint of its input, d(obja*J_n + <ua_in,u_{n+1}>)/du_n.
  * source_grad (device [5][nIC], nullable): the source-term gradient of the
  * same functional is ADDED to it (parameter 'source', apps/adjoint.py:104-106).
+ * bc_grad (device [5][nBoundaryFaces], nullable): the gradient with respect to
+ * the boundary-condition values is ADDED to it, row i / boundary face b
+ * (= face - nInternalFaces) holding the value that sets ghost component i
+ * (parameter ('BCs', field, patch, key), apps/adjoint.py:111-118):
+ *   fixedValue U value -> rows 0-2, T value -> 3, p value -> 4;
+ *   CBC_UPT U0 -> 0-2, T0 -> 3, p0 -> 4;
+ *   CBC_TOTAL_PT direction -> 0-2 (when given), Tt -> 3, pt -> 4.
+ * Faces of other rules are left unchanged.
  * tape (nullable): the tape the forward step from u_n filled; without it the
  * step's stages are recomputed from u_n. */
 int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* source,
-                       void* ua, double obja, void* source_grad, const void* tape, void* stream);
+                       void* ua, double obja, void* source_grad, void* bc_grad, const void* tape,
+                       void* stream);
 
 /* Block-size autotuner for the six stencil kernels (gradient, face flux,
  * stage update, face adjoint, adjoint gather, gradient adjoint).  Times each

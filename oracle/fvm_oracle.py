@@ -28,7 +28,8 @@ How each function is pinned (tests/test_oracle.py unless named):
                            Couette shear in closed form
   accumulate / equation    sum_c V R = 0 on a periodic box; dtc by hand
   step                     conservation, constant-source update, SSP-RK3
-                           third order, autograd vs central differences, and
+                           third order, autograd vs central differences
+                           (w.r.t. the state and w.r.t. boundary values), and
                            the shock tube against the exact solution
                            (tests/golden/shocktube_exact_output.txt, the
                            reference's cases/shockTube/output)
@@ -58,7 +59,12 @@ def T_(a, dtype=DT):
 
 
 def bvalue(v, n, k):
-    """A patch value given uniform (k numbers) or per face (n x k)."""
+    """A patch value given uniform (k numbers) or per face (n x k).  A torch
+    tensor is used as it is (broadcast if uniform), so that autograd of the
+    step reaches boundary-condition values."""
+    if isinstance(v, torch.Tensor):
+        t = v.to(DT)
+        return (t.reshape(1, k).expand(n, k) if t.numel() == k else t.reshape(n, k)) * 1
     a = np.asarray(v, np.float64)
     if a.size == k:
         a = np.tile(a.reshape(1, k), (n, 1))

@@ -46,7 +46,9 @@ class CaseDef(object):
         from adfvm_amd.context import Context
         return Context(self.mesh, self.bcs, self.physics(precision), precision)
 
-    def oracle(self, small=1e-30):
+    def oracle(self, small=1e-30, bcs=None):
+        """The oracle's case; bcs replaces the boundary spec (e.g. with
+        autograd tensors from bc_value_tensors)."""
         from oracle import mesh_oracle, fvm_oracle
         om = mesh_oracle.build(self.md)
         if self.sutherland:
@@ -54,7 +56,8 @@ class CaseDef(object):
         else:
             mu0 = self.mu0
             mu = lambda T: T * 0 + mu0
-        return fvm_oracle.Case(om, self.bcs, GAMMA, CP, PR, mu, self.riemann, self.boundaryRiemann, small,
+        return fvm_oracle.Case(om, self.bcs if bcs is None else bcs, GAMMA, CP, PR, mu, self.riemann,
+                               self.boundaryRiemann, small,
                                self.objective, limiter=self.limiter)
 
 
@@ -167,3 +170,44 @@ def ghost_init(case):
             if spec.get(f, {}).get('type') == 'calculated':
                 arr[s:s + n] = np.broadcast_to(np.asarray(spec[f]['value'], float).reshape(-1, k), (n, k))
     return U, T, p
+
+
+# Boundary-value parameters ('BCs', field, patch, key) per case, with the
+# number of components of each value array.
+def bc_param_cases():
+    def upt_drag():
+        c = mixed_box(4, 'CBC_UPT', sutherland=True, objective=('drag', 'top'))
+        return c, smooth_state(c, 4), [('p', 'outlet', 'U0', 3), ('p', 'outlet', 'T0', 1), ('p', 'outlet', 'p0', 1),
+                                       ('U', 'inlet', 'value', 3), ('T', 'inlet', 'value', 1),
+                                       ('U', 'top', 'value', 3)]
+
+    def totalpt_direction():
+        c = mixed_box(2, 'CBC_TOTAL_PT', 'lf', 'lf', True, objective=('kinetic_energy',))
+        n = c.mesh.boundary['outlet']['nFaces']
+        d = np.random.default_rng(8).normal(size=(n, 3)) * 0.1 + np.array([1., 0., 0.])
+        c.bcs['outlet']['p']['direction'] = d
+        return c, smooth_state(c, 2), [('p', 'outlet', 'Tt', 1), ('p', 'outlet', 'pt', 1),
+                                       ('p', 'outlet', 'direction', 3), ('U', 'inlet', 'value', 3)]
+
+    def totalpt_normal():
+        c = mixed_box(5, 'CBC_TOTAL_PT', objective=('kinetic_energy',))
+        return c, smooth_state(c, 5), [('p', 'outlet', 'Tt', 1), ('p', 'outlet', 'pt', 1),
+                                       ('T', 'inlet', 'value', 1)]
+    return {'upt_drag': upt_drag, 'totalpt_direction': totalpt_direction, 'totalpt_normal': totalpt_normal}
+
+
+def bc_value_tensors(case, params):
+    """A copy of the case's boundary spec whose parameter values are float64
+    torch tensors (nFaces, components) that require grad, for autograd of
+    the oracle; returns (bcs, {param: tensor})."""
+    import copy
+    import torch
+    bcs = copy.deepcopy(case.bcs)
+    ts = {}
+    for field, patch, key, k in params:
+        n = case.mesh.boundary[patch]['nFaces']
+        v = np.broadcast_to(np.asarray(bcs[patch][field][key], np.float64).reshape(-1, k), (n, k))
+        t = torch.tensor(np.ascontiguousarray(v), dtype=torch.float64, requires_grad=True)
+        bcs[patch][field][key] = t
+        ts[(field, patch, key)] = t
+    return bcs, ts

@@ -187,3 +187,90 @@ def test_tape_equals_recompute_bitwise(name):
         ctx.adjoint_step(u0, dt, a, obja=1.0, tape=tape)
         ctx.adjoint_step(u0, dt, b, obja=1.0)
         assert torch.equal(a, b)
+
+
+BC_CASES = cf.bc_param_cases()
+
+
+def oracle_bc_vjp(case, u, dt, ua, obja, params, rnd=lambda a: a):
+    """d(<ua, step(u)> + obja J)/d(value) for each boundary-value parameter,
+    by the oracle's autograd; rnd rounds the values as the device holds them."""
+    bcs, ts = cf.bc_value_tensors(case, params)
+    with torch.no_grad():
+        for t in ts.values():
+            t.copy_(torch.as_tensor(rnd(t.numpy())))
+    oc = case.oracle(bcs=bcs)
+    x = torch.tensor(u, dtype=torch.float64)
+    gi = [torch.as_tensor(a, dtype=torch.float64) for a in cf.ghost_init(case)]
+    r, rU, rE, dtc, obj = oc.step(x[0:1].T, x[1:4].T, x[4:5].T, dt, ghostInit=gi)
+    L = (torch.cat([r.T, rU.T, rE.T]) * torch.as_tensor(ua)).sum() + obja * obj
+    grads = torch.autograd.grad(L, [ts[p[:3]] for p in params])
+    return {p[:3]: g.numpy() for p, g in zip(params, grads)}
+
+
+@pytest.mark.parametrize('name', sorted(BC_CASES))
+def test_bc_value_gradient_matches_autograd(name):
+    """The boundary-value gradients an adjoint step adds to bc_grad
+    (parameter ('BCs', field, patch, key)) against the oracle's autograd:
+    fixedValue U and T, CBC_UPT U0/T0/p0, CBC_TOTAL_PT Tt/pt/direction, with
+    the drag objective on a fixedValue wall."""
+    case, u, params = BC_CASES[name]()
+    ua = scaled_rbar(case, u, 4) * 1e3
+    dt = 1e-4
+    obja = 1.0 if case.objective else 0.0
+    for prec, tol in ((np.float64, 1e-11), (np.float32, 2e-4)):
+        r = lambda a: a.astype(prec).astype(np.float64)
+        ref = oracle_bc_vjp(case, r(u), dt, r(ua), obja, params, rnd=r)
+        ctx = case.context(prec)
+        ud = torch.as_tensor(u.astype(prec), device='cuda').contiguous()
+        uad = torch.as_tensor(ua.astype(prec), device='cuda').contiguous()
+        bg = ctx.bc_state()
+        ctx.adjoint_step(ud, dt, uad, obja=obja, bc_grad=bg)
+        bgn = bg.cpu().numpy().astype(np.float64)
+        for field, patch, key, k in params:
+            rows, faces = ctx.bc_param(field, patch, key)
+            got = bgn[rows, faces].T
+            err = rel_l2(got, ref[(field, patch, key)])
+            print('%s %s %s/%s/%s rel-L2 %.2e' % (name, np.dtype(prec).name, field, patch, key, err))
+            assert err < tol, (field, patch, key, err)
+        # faces of rules without values stay zero
+        for patch, info in ctx._bcinfo.items():
+            if info['cbc'] == 0 and all(t != 'fixedValue' for t in info['types'].values()):
+                assert not bgn[:, info['start']:info['start'] + info['n']].any(), patch
+
+
+def test_bc_value_gradient_vs_central_fd_multistep():
+    """d/d eps of sum_n J_n over 5 steps (drag on the no-slip wall) along
+    random perturbations of the CBC_UPT outlet's U0 and the inlet's fixed
+    U: the adjoint sweep's bc_grad against central differences of the HIP
+    primal with the values replaced on the device (set_bc_value), FP64."""
+    from adfvm_amd import adjoint
+    case, u, params = BC_CASES['upt_drag']()
+    ctx = case.context(np.float64)
+    nsteps, dt = 5, 1e-4
+    u0 = torch.as_tensor(u, device='cuda').contiguous()
+    bg = ctx.bc_state()
+    adjoint.sweep(ctx, u0, [dt] * nsteps, schedule='store_all', bc_grad=bg)
+    rng = np.random.default_rng(21)
+    for field, patch, key, k in (('p', 'outlet', 'U0', 3), ('U', 'inlet', 'value', 3), ('p', 'outlet', 'p0', 1)):
+        n = case.mesh.boundary[patch]['nFaces']
+        base = np.broadcast_to(np.asarray(case.bcs[patch][field][key], np.float64).reshape(-1, k), (n, k)).copy()
+        dv = rng.normal(size=(n, k)) * (np.abs(base).mean() + 1.0)
+        rows, faces = ctx.bc_param(field, patch, key)
+        adj = float((bg[rows, faces].T.cpu().numpy() * dv).sum())
+
+        def J(s, eps):
+            ctx.set_bc_value(field, patch, key, base + s * eps * dv)
+            ud = u0.clone()
+            obj = torch.zeros(1, dtype=torch.float64, device='cuda')
+            tot = 0.0
+            for _ in range(nsteps):
+                ctx.primal_step(ud, dt, obj_out=obj)
+                tot += obj.item()
+            return tot
+        eps = 1e-4
+        fd = (8 * (J(1, eps) - J(-1, eps)) - (J(2, eps) - J(-2, eps))) / (12 * eps)
+        ctx.set_bc_value(field, patch, key, base)
+        rel = abs(adj - fd) / abs(fd)
+        print('bc fd %s/%s/%s: adjoint %.12e fd %.12e rel %.2e' % (field, patch, key, adj, fd, rel))
+        assert rel <= 1e-6, (field, patch, key, rel)

@@ -209,6 +209,34 @@ def test_apps_cylinder_drag(tmp_path, monkeypatch):
     assert abs(fdSens - adjSens) / abs(fdSens) < 1e-3
 
 
+def test_apps_cylinder_bc_value(tmp_path, monkeypatch):
+    """The same workflow with a boundary-condition parameter, as the
+    reference's templates/3d_cylinder.py ('BCs', field, patch, key): the
+    drag's sensitivity to the cylinder wall's fixed velocity
+    (templates/cylinder_bc_test.py); the perturb run changes the value on
+    the device, the adjoint run sums the boundary-value gradient;
+    |fd - adj| / |fd| < 1e-3."""
+    from adfvm_amd.apps import problem, adjoint as adjoint_app
+    case = str(tmp_path / 'cylinder')
+    monkeypatch.setenv('ADFVM_CASE_DIR', case)
+    monkeypatch.chdir(tmp_path)
+    tpl = os.path.join(REPO, 'adfvm_amd', 'templates', 'cylinder_bc_test.py')
+    old = config.precision
+    try:
+        for app, args in ((problem, [tpl, 'orig']), (problem, [tpl, 'perturb']), (adjoint_app, [tpl])):
+            print('running', app.__name__, ' '.join(args[1:]), flush=True)
+            app.main(args + ['--gpu_double'])
+    finally:
+        config.set_precision(old)
+    with open(os.path.join(case, 'objective.txt')) as f:
+        data = f.readlines()
+    print(''.join(data))
+    fdSens = float(data[1].split(' ')[-2])
+    adjSens = float(data[2].split(' ')[-2])
+    assert fdSens != 0
+    assert abs(fdSens - adjSens) / abs(fdSens) < 1e-3
+
+
 @pytest.mark.parametrize('prec', [np.float64, np.float32])
 def test_shocktube_exact(tmp_path, prec):
     """cases/shockTube (500 cells on [-5, 5], the reference's binary 0/

@@ -402,3 +402,49 @@ def test_shock_tube_against_exact_solution():
     pp = (G - 1) * (u[2][:, 0].numpy() - 0.5 * rho * Ux ** 2)
     for got, ref, tol in ((rho, ex[:, 0], 0.02), (pp, ex[:, 2], 0.02), (Ux, ex[:, 1], 0.05)):
         assert np.abs(got - ref).sum() / np.abs(ref).sum() < tol
+
+
+@pytest.mark.parametrize('name', sorted(__import__('tests.casefactory', fromlist=['x']).bc_param_cases()))
+def test_step_bc_value_gradient_autograd_vs_central_differences(name):
+    """Autograd of one step's functional <w, step(u)> + J with respect to
+    boundary-condition values (fixedValue U/T, CBC_UPT U0/T0/p0,
+    CBC_TOTAL_PT Tt/pt/direction; parameter ('BCs', field, patch, key),
+    apps/adjoint.py:111-118) equals central differences of the step along a
+    random direction of each value array."""
+    from tests import casefactory as cf
+    case, u, params = cf.bc_param_cases()[name]()
+    rng = np.random.default_rng(12)
+    w = rng.normal(size=u.shape) * np.abs(u).mean(1, keepdims=True) * 1e-3
+    gi = [T_(a) for a in cf.ghost_init(case)]
+    obja = 0.7 if case.objective else 0.0
+
+    def F(bcs):
+        oc = case.oracle(bcs=bcs)
+        x = T_(u)
+        # dt at CFL ~0.2 (cells 0.2 m, |U| + c ~ 380 m/s): a boundary value
+        # moves the new state by O(dt), well above its rounding
+        r, rU, rE, dtc, obj = oc.step(x[0:1].T, x[1:4].T, x[4:5].T, 1e-4, ghostInit=gi)
+        return (torch.cat([r.T, rU.T, rE.T]) * T_(w)).sum() + obja * obj
+
+    bcs, ts = cf.bc_value_tensors(case, params)
+    grads = torch.autograd.grad(F(bcs), [ts[p[:3]] for p in params])
+    for (field, patch, key, k), g in zip(params, grads):
+        v0 = ts[(field, patch, key)].detach().numpy()
+        dv = rng.normal(size=v0.shape) * (np.abs(v0).mean() + 1.0)
+        ad = float((g * T_(dv)).sum())
+
+        def at(s, eps):
+            b, _ = cf.bc_value_tensors(case, [])
+            b[patch][field][key] = v0 + s * eps * dv
+            return float(F(b))
+
+        # fourth-order central differences at two steps (relative to the
+        # value's scale): the larger wins where rounding of the new state
+        # dominates, the smaller where a kink of the LF wave-speed maximum
+        # lies within the larger step
+        errs = []
+        for eps in (1e-3, 2e-4):
+            fd = (8 * (at(1, eps) - at(-1, eps)) - (at(2, eps) - at(-2, eps))) / (12 * eps)
+            errs.append(abs(ad - fd) / abs(fd))
+        print(name, field, patch, key, 'rel %.1e %.1e' % tuple(errs))
+        assert min(errs) <= 1e-7, (field, patch, key, ad, errs)
