@@ -45,6 +45,22 @@ class CaseSetup(object):
         rho, rhoU, rhoE = conservative(self.U, self.T, self.p)
         return np.concatenate([rho.T, rhoU.T, rhoE.T])
 
+    def context(self, precision, riemann='eulerRoe', boundaryRiemann='eulerRoe', objective=None):
+        """A device context for this case without a case directory (what
+        RCF.compile builds from the case files).  objective: None,
+        ('kinetic_energy',) or ('drag', patch)."""
+        from . import BCs, _lib, config
+        from .context import Context
+        flux = {'eulerRoe': _lib.RIEMANN_ROE, 'eulerLaxFriedrichs': _lib.RIEMANN_LF}
+        ph = dict(gamma=GAMMA, Cp=CP, Pr=0.7, mu=(_lib.MU_CONSTANT, float(self.mu), 0., 0.),
+                  riemann=flux[riemann], boundaryRiemann=flux[boundaryRiemann], small=config.small_for(precision),
+                  integrator=_lib.RK_SSPRK3)
+        if objective is not None:
+            ph['objective'] = {'kinetic_energy': _lib.OBJ_KINETIC_ENERGY, 'drag': _lib.OBJ_DRAG}[objective[0]]
+            if objective[0] == 'drag':
+                ph['objectivePatch'] = objective[1]
+        return Context(self.mesh, BCs.validate(self.mesh, self.boundary), ph, precision)
+
 
 def taylor_green(n, seed=0, rcm=True, mach=0.1, rho0=1.2, T0=300., Re=1600.):
     md = meshgen.permute_and_rcm(meshgen.periodic_box(n), seed=seed, rcm=rcm)

@@ -7,9 +7,8 @@ instances share the GPU instead of each under-filling it, and each instance's
 arithmetic is exactly what it would be alone (own cell order, own slot
 tables, own geometry).  The context evaluates the objective per instance
 (`Context.set_segments`).  The reverse sweep is the uniform checkpoint
-schedule; the forward segment sweep and the segment re-run + reverse are
-each captured once into a HIP graph (torch.cuda.CUDAGraph) and replayed, so
-a whole segment is one graph launch.  Instances are independent, so the
+schedule with HIP-graph segments (`graphs.graph_sweep`): a whole segment is
+one graph launch.  Instances are independent, so the
 adjoint of the summed objective gives every instance its own gradient.
 
     ens = TaylorGreenEnsemble(n_instances=64, sizes=(32, 48, 64, 96), seed=0)
@@ -63,6 +62,7 @@ class TaylorGreenEnsemble(object):
         self.ctx.set_segments(self.offsets)
         self.nSeg = len(self.offsets) - 1
         self.u0 = torch.as_tensor(u0, dtype=self.ctx.tdtype, device='cuda').contiguous()
+        self._sweeps = {}
 
     @property
     def n_cells(self):
@@ -73,58 +73,8 @@ class TaylorGreenEnsemble(object):
         energy (on each step's stage-1 state).  Returns (J (nSeg,) FP64
         host array, ua: d(sum_k J_k)/du0 on the device, i.e. each instance's
         own gradient)."""
-        torch, ctx = self.torch, self.ctx
-        if n_steps % interval:
-            raise ValueError('n_steps must be a multiple of interval')
-        nseg_time = n_steps // interval
-        dt = self.dt
-        u = torch.empty_like(self.u0)
-        obj = torch.zeros((interval, self.nSeg), dtype=torch.float64, device='cuda')
-        states = torch.empty((interval,) + tuple(self.u0.shape), dtype=ctx.tdtype, device='cuda')
-        tps = ctx.new_tapes(interval) if tapes else None
-        ua = torch.zeros_like(self.u0)
-
-        J = torch.zeros(self.nSeg, dtype=torch.float64, device='cuda')
-
-        def forward_segment():
-            for k in range(interval):
-                ctx.primal_step(u, dt, obj_out=obj[k])
-                J.add_(obj[k])                  # step order, as a single run accumulates
-
-        def reverse_segment():
-            for k in range(interval):
-                states[k].copy_(u)
-                ctx.primal_step(u, dt, tape=None if tps is None else tps[k])
-            for k in reversed(range(interval)):
-                ctx.adjoint_step(states[k], dt, ua, obja=1.0, tape=None if tps is None else tps[k])
-
-        if graphs:
-            u.copy_(self.u0)
-            g_fwd, g_rev = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
-            side = torch.cuda.Stream()
-            side.wait_stream(torch.cuda.current_stream())
-            with torch.cuda.stream(side):            # warm-up outside capture
-                forward_segment()
-                reverse_segment()
-            torch.cuda.current_stream().wait_stream(side)
-            with torch.cuda.graph(g_fwd):
-                forward_segment()
-            with torch.cuda.graph(g_rev):
-                reverse_segment()
-            run_fwd, run_rev = g_fwd.replay, g_rev.replay
-        else:
-            run_fwd, run_rev = forward_segment, reverse_segment
-        # forward sweep: objective, and the state at every segment start
-        u.copy_(self.u0)
-        J.zero_()
-        ua.zero_()
-        checkpoints = []
-        for s in range(nseg_time):
-            checkpoints.append(u.clone())
-            run_fwd()
-        # reverse sweep, last segment first
-        for s in reversed(range(nseg_time)):
-            u.copy_(checkpoints.pop())
-            run_rev()
-        torch.cuda.synchronize()
-        return J.cpu().numpy(), ua
+        from .graphs import GraphSweep
+        key = (interval, graphs, tapes)
+        if key not in self._sweeps:                 # captured once, replayed by later runs
+            self._sweeps[key] = GraphSweep(self.ctx, self.dt, interval, 1.0, graphs, tapes)
+        return self._sweeps[key].run(self.u0, n_steps)

@@ -1,0 +1,104 @@
+"""Primal + adjoint sweep with HIP-graph segments.
+
+For fixed-dt runs the work between two checkpoints is the same every time:
+`interval` primal steps forward (objective accumulated on the device), and
+for the reverse the same steps re-run keeping their start states and tapes,
+then `interval` adjoint steps.  Each of the two is captured once into a HIP
+graph (torch.cuda.CUDAGraph) and replayed per segment, so a segment costs
+one graph launch instead of a few hundred kernel launches -- what matters on
+meshes small enough for launch overhead to show (the cylinder O-grid, small
+ensemble members).  The reverse sweep is the uniform checkpoint schedule of
+apps/adjoint.py:205-217 (checkpoints every writeInterval steps).
+"""
+import numpy as np
+
+
+class GraphSweep(object):
+    """Buffers and captured graphs for repeated sweeps of one context at one
+    dt, checkpoint interval and objective weight.  The graphs are captured
+    on the first run (after one eager warm-up segment on a side stream) and
+    replayed by every later run.  With bc_grad, the boundary-value
+    gradient d(obja * J)/d(BC values) is accumulated into self.bc_grad
+    ((5, nBoundaryFaces), Context.bc_param locates a parameter in it)."""
+
+    def __init__(self, ctx, dt, interval, obja=1.0, graphs=True, tapes=True, source=None, bc_grad=False):
+        torch = ctx.torch
+        self.torch, self.ctx = torch, ctx
+        self.dt, self.interval, self.obja, self.source = float(dt), int(interval), float(obja), source
+        self.graphs = graphs
+        nseg = getattr(ctx, 'nSeg', 1)
+        shape = (5, ctx.nIC)
+        self.u = torch.empty(shape, dtype=ctx.tdtype, device='cuda')
+        self.obj = torch.zeros((interval, nseg), dtype=torch.float64, device='cuda')
+        self.states = torch.empty((interval,) + shape, dtype=ctx.tdtype, device='cuda')
+        self.tps = ctx.new_tapes(interval) if tapes else None
+        self.ua = torch.zeros(shape, dtype=ctx.tdtype, device='cuda')
+        self.J = torch.zeros(nseg, dtype=torch.float64, device='cuda')
+        self.bc_grad = ctx.bc_state() if bc_grad else None
+        self._fwd = self._rev = None
+
+    def _forward_segment(self):
+        ctx, u, dt = self.ctx, self.u, self.dt
+        for k in range(self.interval):
+            ctx.primal_step(u, dt, source=self.source, obj_out=self.obj[k])
+            self.J.add_(self.obj[k])            # step order, as a single run accumulates
+
+    def _reverse_segment(self):
+        ctx, u, dt, tps = self.ctx, self.u, self.dt, self.tps
+        for k in range(self.interval):
+            self.states[k].copy_(u)
+            ctx.primal_step(u, dt, source=self.source, tape=None if tps is None else tps[k])
+        for k in reversed(range(self.interval)):
+            ctx.adjoint_step(self.states[k], dt, self.ua, obja=self.obja, source=self.source,
+                             bc_grad=self.bc_grad, tape=None if tps is None else tps[k])
+
+    def _capture(self, u0):
+        torch = self.torch
+        self.u.copy_(u0)
+        g_fwd, g_rev = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):            # warm-up outside capture
+            self._forward_segment()
+            self._reverse_segment()
+        torch.cuda.current_stream().wait_stream(side)
+        with torch.cuda.graph(g_fwd):
+            self._forward_segment()
+        with torch.cuda.graph(g_rev):
+            self._reverse_segment()
+        self._fwd, self._rev = g_fwd, g_rev
+
+    def run(self, u0, n_steps):
+        """Forward n_steps steps from the device state u0 (unchanged), then
+        the reverse sweep.  Returns (J, ua): J the FP64 sum over steps of the
+        objective (one entry per cell segment of the context, on the host),
+        ua = d(obja * J)/du0 on the device (a buffer reused by the next
+        run)."""
+        if n_steps % self.interval:
+            raise ValueError('n_steps must be a multiple of interval')
+        if self.graphs and self._fwd is None:
+            self._capture(u0)
+        run_fwd = self._fwd.replay if self.graphs else self._forward_segment
+        run_rev = self._rev.replay if self.graphs else self._reverse_segment
+        u = self.u
+        # forward sweep: objective, and the state at every segment start
+        u.copy_(u0)
+        self.J.zero_()
+        self.ua.zero_()
+        if self.bc_grad is not None:
+            self.bc_grad.zero_()
+        checkpoints = []
+        for s in range(n_steps // self.interval):
+            checkpoints.append(u.clone())
+            run_fwd()
+        # reverse sweep, last segment first
+        while checkpoints:
+            u.copy_(checkpoints.pop())
+            run_rev()
+        self.torch.cuda.synchronize()
+        return np.asarray(self.J.cpu().numpy()), self.ua
+
+
+def graph_sweep(ctx, u0, dt, n_steps, interval, obja=1.0, graphs=True, tapes=True, source=None):
+    """One sweep (GraphSweep(...).run(u0, n_steps))."""
+    return GraphSweep(ctx, dt, interval, obja, graphs, tapes, source).run(u0, n_steps)

@@ -177,6 +177,61 @@ def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
             'objective_range': [float(J.min()), float(J.max())]}
 
 
+def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, ratio=1.02):
+    """2-D cylinder wake: O-grid of n_theta x n_r x 1 hex cells, radial
+    spacing growing by `ratio` from r = 0.5 D to 20 D, Mach 0.2, Re = 100,
+    isothermal no-slip wall, CBC_UPT far field, Lax-Friedrichs at the
+    boundaries.  n_steps fixed-dt steps at CFL 0.5 (dt = 2 CFL/dtc of the
+    initial state, solver.py:359-366) primal plus the adjoint of the mean
+    drag, uniform checkpoints every `interval` steps, each checkpoint
+    segment one HIP graph; FP32 kernels with FP64 objective accumulation,
+    and the same run in FP64 for the error and speed ratio."""
+    from adfvm_amd import cases
+    from adfvm_amd.graphs import GraphSweep
+    t0 = time.time()
+    s = cases.cylinder(n_theta, n_r, ratio=ratio)
+    nC = s.mesh.nInternalCells
+    print('[bench] cylinder %dx%d, %d cells, built in %.1fs' % (n_theta, n_r, nC, time.time() - t0),
+          file=sys.stderr, flush=True)
+    res, dt, tuned = {}, None, None
+    for prec in (np.float32, np.float64):
+        ctx = s.context(prec, boundaryRiemann='eulerLaxFriedrichs', objective=('drag', 'cylinder'))
+        u0 = torch.as_tensor(s.state(), dtype=ctx.tdtype, device='cuda').contiguous()
+        if dt is None:
+            dtc = torch.zeros(1, dtype=torch.float64, device='cuda')
+            ctx.primal_step(u0.clone(), 1e-12, dtc_out=dtc)
+            dt = 2 * 0.5 / dtc.item()
+        if prec == np.float32:
+            tuned = ctx.autotune(u0, reps=3)
+        sweep = GraphSweep(ctx, dt, interval, obja=1.0 / n_steps)
+        sweep.run(u0, interval)                    # capture + one segment
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J, ua = sweep.run(u0, n_steps)
+        t = time.perf_counter() - t1
+        res[prec] = (J[0] / n_steps, ua.double().cpu().numpy(), t)
+        print('[bench] cylinder %s: %.3f s, mean drag %.10e' % (np.dtype(prec).name, t, J[0] / n_steps),
+              file=sys.stderr, flush=True)
+    d32, g32, t32 = res[np.float32]
+    d64, g64, t64 = res[np.float64]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {'metric': 'cell-updates/s primal+adjoint, FP32 vs FP64, % HBM roofline, L2 err vs FP64',
+            'value': nC * n_steps / t32, 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': n_steps, 'warmup': interval,
+            'ms_per_step': t32 / n_steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': 'f32', 'data': 'synthetic (uniform free stream, impulsive start)',
+            'config': {'workload': '2D cylinder wake, O-grid %d x %d x 1, radial stretch %g from r = 0.5 D to 20 D, '
+                                   'Mach 0.2, Re 100, isothermal no-slip wall, CBC_UPT far field, LF boundary '
+                                   'flux; %d steps primal + adjoint of the mean drag, checkpoints every %d steps, '
+                                   'HIP-graph segments' % (n_theta, n_r, ratio, n_steps, interval),
+                       'n_cells': nC, 'dt': dt},
+            'block_sizes': {k: v[0] for k, v in tuned.items()},
+            'value_fp64': nC * n_steps / t64, 'fp32_over_fp64': t64 / t32,
+            'mean_drag_fp32': d32, 'mean_drag_fp64': d64, 'mean_drag_rel_err_fp32_vs_fp64': abs(d32 - d64) / abs(d64),
+            'grad_rel_l2_fp32_vs_fp64': {'rho': rel(g32[0:1], g64[0:1]), 'rhoU': rel(g32[1:4], g64[1:4]),
+                                         'rhoE': rel(g32[4:5], g64[4:5])},
+            'finite': bool(np.isfinite(g32).all() and np.isfinite(g64).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -190,6 +245,9 @@ def main():
     ap.add_argument('--ensemble', action='store_true',
                     help='instead: the batched ensemble (64 Taylor-Green instances of 32^3..96^3, 200 steps '
                          'primal+adjoint, checkpoints every 20, HIP-graph segments)')
+    ap.add_argument('--cylinder', action='store_true',
+                    help='instead: the 2D cylinder wake (512 x 256 O-grid, 5000 steps primal + adjoint of the mean '
+                         'drag, checkpoints every 100, HIP-graph segments), FP32 and FP64')
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB five times (counter calibration)')
     args = ap.parse_args()
@@ -203,6 +261,9 @@ def main():
     torch.cuda.set_device(0)
     if args.ensemble:
         print(json.dumps(ensemble_bench(torch)))
+        return
+    if args.cylinder:
+        print(json.dumps(cylinder_bench(torch)))
         return
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4

@@ -148,3 +148,42 @@ def test_dot_and_energy_fp64_accumulation():
     assert abs(d - ref) <= 1e-12 * abs(ref)
     e = ctx.adjoint_energy(b, a).item()
     assert np.isfinite(e) and e > 0
+
+
+def test_sod_1000_cells_200_steps_against_oracle():
+    """The CPU-runnable shock tube: 1000 cells on [0, 1], left p = 1e5 Pa,
+    T = 348.4 K, right p = 1e4 Pa, T = 278.7 K, inviscid Roe, second-order
+    reconstruction (limited: unlimited, the first step's face pressure goes
+    negative), SSP-RK3, 200 steps at CFL 0.5 under the reference's rule
+    dt = min(2 CFL/dtc, 1.2 dt) (solver.py:359-366).  The HIP steps set the
+    dt sequence; the oracle replays it.  FP64 relative L2 per conserved
+    variable <= 1e-12; FP32 against the same FP64 oracle <= 1e-4 (limiter
+    branches at near-ties can differ between precisions)."""
+    case = cf.sod(1000)
+    u0 = cf.sod_state(case)
+    nsteps, cfl = 200, 0.5
+    results = {}
+    for prec in (np.float64, np.float32):
+        ctx = case.context(prec)
+        ud = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
+        dtc = torch.zeros(1, dtype=torch.float64, device='cuda')
+        dt, dts = 1e-8, []
+        for _ in range(nsteps):
+            ctx.primal_step(ud, dt, dtc_out=dtc)
+            dts.append(dt)
+            dt = min(2 * cfl / dtc.item(), dt * 1.2)
+        results[prec] = (ud.cpu().numpy().astype(np.float64), dts)
+        assert np.isfinite(results[prec][0]).all()
+    oc = case.oracle()
+    x = torch.as_tensor(u0)
+    for dt in results[np.float64][1]:
+        r, rU, rE, _, _ = oc.step(x[0:1].T, x[1:4].T, x[4:5].T, dt)
+        x = torch.cat([r.T, rU.T, rE.T])
+    u_o = x.numpy()
+    print('sod 1000: t_end %.6e' % sum(results[np.float64][1]))
+    for prec, tol in ((np.float64, 1e-12), (np.float32, 1e-4)):
+        un = results[prec][0]
+        errs = [rel_l2(un[s], u_o[s]) for s in (slice(0, 1), slice(1, 4), slice(4, 5))]
+        print('sod 1000 %s 200 steps rel-L2 rho/rhoU/rhoE %s' % (np.dtype(prec).name,
+              ' '.join('%.2e' % e for e in errs)))
+        assert max(errs) < tol, errs

@@ -266,3 +266,63 @@ def test_shocktube_exact(tmp_path, prec):
     errs = [np.abs(a - ref).sum() / np.abs(ref).sum() for a, ref in ((rho, d[:, 2]), (p, d[:, 3]), (U, d[:, 4]))]
     print('shock tube %s: L1 rel rho %.2e p %.2e U %.2e' % (np.dtype(prec).name, *errs))
     assert errs[0] < 0.01 and errs[1] < 0.01 and errs[2] < 0.01
+
+
+def test_cylinder_wake_full_size_fp64():
+    """The cylinder wake at full size: 512 x 256 O-grid (radial stretch 1.02
+    from r = 0.5 to 20 D), Mach 0.2, Re = 100, isothermal no-slip wall,
+    CBC_UPT far field, LF at the boundaries, 5000 steps at CFL 0.5, the
+    mean drag's adjoint by the HIP-graph checkpointed sweep (every 100
+    steps), FP64.  Its gradients along a random initial-velocity
+    perturbation and along a random change of the wall velocity (a
+    boundary-value parameter) against fourth-order central differences of
+    the HIP primal: relative difference <= 1e-6."""
+    from adfvm_amd.graphs import GraphSweep
+    s = cases.cylinder(512, 256, ratio=1.02)
+    ctx = s.context(np.float64, boundaryRiemann='eulerLaxFriedrichs', objective=('drag', 'cylinder'))
+    u0 = torch.as_tensor(s.state(), dtype=ctx.tdtype, device='cuda').contiguous()
+    dtc = torch.zeros(1, dtype=torch.float64, device='cuda')
+    ctx.primal_step(u0.clone(), 1e-12, dtc_out=dtc)
+    dt = 2 * 0.5 / dtc.item()
+    n, interval = 5000, 100
+    sweep = GraphSweep(ctx, dt, interval, obja=1.0 / n, bc_grad=True)
+    J, ua = sweep.run(u0, n)
+    print('cylinder 512x256: dt %.4e, %d steps, mean drag %.12e' % (dt, n, J[0] / n), flush=True)
+    assert np.isfinite(J).all() and bool(torch.isfinite(ua).all())
+    rng = np.random.default_rng(17)
+    nIC = ctx.nIC
+    x = s.mesh.cellCentres[:nIC]
+    # smooth initial-velocity perturbation upstream of the cylinder
+    g = np.exp(-((x[:, 0] + 1.0) ** 2 + (x[:, 1] - 0.3) ** 2) / 0.1)
+    v = np.zeros((5, nIC))
+    u0n = s.state()
+    v[1] = u0n[0] * g                               # rhoU_x += rho * g, at fixed rho and p
+    v[4] = u0n[1] * g
+    rows, faces = ctx.bc_param('U', 'cylinder', 'value')
+    nw = faces.stop - faces.start
+    dw = rng.normal(size=(nw, 3)) * np.array([1., 1., 0.])
+    ad_u = float((ua.cpu().numpy() * v).sum())
+    ad_w = float((sweep.bc_grad[rows, faces].T.cpu().numpy() * dw).sum())
+
+    def mean_drag(u_init):
+        u = torch.as_tensor(u_init, device='cuda').contiguous()
+        obj = torch.zeros(1, dtype=torch.float64, device='cuda')
+        tot = torch.zeros(1, dtype=torch.float64, device='cuda')
+        for _ in range(n):
+            ctx.primal_step(u, dt, obj_out=obj)
+            tot += obj
+        return tot.item() / n
+
+    def fd4(f, eps):
+        return (8 * (f(eps) - f(-eps)) - (f(2 * eps) - f(-2 * eps))) / (12 * eps)
+    fd_u = fd4(lambda e: mean_drag(u0n + e * v), 1e-3)
+
+    def with_wall(e):
+        ctx.set_bc_value('U', 'cylinder', 'value', e * dw)
+        return mean_drag(u0n)
+    fd_w = fd4(with_wall, 1e-3)
+    ctx.set_bc_value('U', 'cylinder', 'value', np.zeros((nw, 3)))
+    for name, ad, fd in (('initial velocity', ad_u, fd_u), ('wall velocity', ad_w, fd_w)):
+        rel = abs(ad - fd) / abs(fd)
+        print('cylinder %s: adjoint %.12e fd %.12e rel %.2e' % (name, ad, fd, rel), flush=True)
+        assert rel <= 1e-6, (name, rel)
