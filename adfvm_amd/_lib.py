@@ -16,6 +16,7 @@ BC_CYCLIC, BC_ZERO_GRADIENT, BC_SYMMETRY, BC_FIXED_VALUE, BC_CALCULATED = 0, 1, 
 FLUX_COUPLED, FLUX_CHARACTERISTIC, FLUX_BOUNDARY = 0, 1, 2
 CBC_NONE, CBC_UPT, CBC_TOTAL_PT = 0, 1, 2
 BCV_U, BCV_T, BCV_P, BCV_TT, BCV_PT, BCV_DIRECTION = 0, 1, 2, 3, 4, 5
+VISC_UNIFORM, VISC_ABARBANEL, VISC_TURKEL = 0, 1, 2
 RIEMANN_ROE, RIEMANN_LF = 0, 1
 MU_CONSTANT, MU_SUTHERLAND = 0, 1
 RK_SSPRK3, RK_EULER = 0, 1
@@ -90,6 +91,7 @@ SIGNATURES = {
     'adfvm_tape_bytes': (C.c_int64, [vp]),
     'adfvm_adjoint_step': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp, vp, vp]),
     'adfvm_set_bc_values': (C.c_int, [vp, C.c_int32, C.c_int32, vp, C.c_int64, vp]),
+    'adfvm_adjoint_viscosity': (C.c_int, [vp, vp, vp, C.c_double, C.c_int32, C.c_double, C.c_int32, vp, vp]),
     'adfvm_gradients': (C.c_int, [vp, vp, vp, vp]),
     'adfvm_residual': (C.c_int, [vp, vp, vp, vp, vp]),
     'adfvm_residual_vjp': (C.c_int, [vp, vp, vp, vp, vp]),

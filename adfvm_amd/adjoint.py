@@ -112,8 +112,17 @@ class Adjoint(object):
     its checkpoints (time directories every writeInterval steps) and its
     time-step log."""
 
-    def __init__(self, primal, scaling=0.0):
+    def __init__(self, primal, scaling=0.0, viscosityType=None, viscosityScaler=None, viscousIterations=1000):
+        """scaling, viscosityType, viscosityScaler: the case file's adjParams
+        (apps/adjoint.py:46-48).  With a viscosityType ('uniform',
+        'abarbanel', 'turkel') every viscousInterval-th reversed step is
+        followed by the adjoint-viscosity smoothing (apps/adjoint.py:127-140,
+        287-291) of viscousIterations Jacobi sweeps; viscosityScaler is
+        used only by the reference's CPU/PETSc variant."""
         self.primal = primal
+        self.viscosityType = viscosityType
+        self.viscosityScaler = viscosityScaler
+        self.viscousIterations = viscousIterations
         self.mesh = primal.mesh
         self.names = [name + 'a' for name in primal.names]
         self.dimensions = primal.dimensions
@@ -185,7 +194,7 @@ class Adjoint(object):
 
     def run(self, checkpointData, primalData, nSteps, writeInterval, perturb=(), parameters=('source',),
             source=None, sampleInterval=1, reportInterval=1, avgStart=0, runCheckpoints=2 ** 30, tapes=True,
-            verbose=False):
+            verbose=False, viscousInterval=1):
         """The reverse sweep of apps/adjoint.py:176-404 for the parameter
         parameters[0]: 'source' or a boundary-condition value
         ('BCs', field, patch, key) (apps/adjoint.py:104-118); returns the
@@ -258,6 +267,9 @@ class Adjoint(object):
                 k = writeInterval - r
                 ctx.adjoint_step(states[k], dts[k], ua, obja=obja, source=primal.source, source_grad=sgrad,
                                  bc_grad=bgrad, tape=tps[k])
+                if self.viscosityType and r % viscousInterval == 0:       # apps/adjoint.py:250,288-289
+                    ctx.adjoint_viscosity(states[k], ua, dts[k], self.viscosityType, self.scaling,
+                                          iterations=self.viscousIterations)
                 if r % reportInterval == 0:
                     ctx.adjoint_energy(ua, states[k], out=energy)
                     energyTimeSeries.append(energy.item() ** 0.5)

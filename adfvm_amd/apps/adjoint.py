@@ -22,10 +22,7 @@ def main(argv=None):
     user = parser.parse_args(rest)
     primal = ns['primal']
     adjParams = ns['adjParams']
-    if adjParams[1]:
-        raise SystemExit('adjoint viscosity stabilisation (adjParams[1]=%r) is not supported' % (adjParams[1],))
-
-    adjoint = Adjoint(primal, scaling=adjParams[0])
+    adjoint = Adjoint(primal, scaling=adjParams[0], viscosityType=adjParams[1], viscosityScaler=adjParams[2])
     primal.readFields(ns['startTime'])
     primal.compile()
     adjoint.createFields()
@@ -35,7 +32,7 @@ def main(argv=None):
     result = adjoint.run(*data, nSteps=ns['nSteps'], writeInterval=ns['writeInterval'], perturb=ns['perturb'],
                          parameters=ns['parameters'], source=ns['source'], sampleInterval=ns['sampleInterval'],
                          reportInterval=ns['reportInterval'], avgStart=ns['avgStart'],
-                         runCheckpoints=ns['runCheckpoints'])
+                         runCheckpoints=ns['runCheckpoints'], viscousInterval=ns['viscousInterval'])
     if adjoint.finished:
         writeResult(ns, 'adjoint', result, str(adjoint.scaling), adjoint.sensTimeSeriesFile)
 

@@ -214,6 +214,23 @@ class Context(object):
                                               float(obja), _dptr(source_grad), _dptr(bc_grad), _dptr(tape),
                                               _stream(stream)))
 
+    def adjoint_viscosity(self, u_n, ua, dt, vtype, scaling, iterations=1000, m2norm_out=None, stream=None):
+        """Adjoint-viscosity smoothing of ua in place after the adjoint step
+        from u_n (apps/adjoint.py:127-140, postpro.py:491-719): vtype
+        'uniform', 'abarbanel' or 'turkel'; scaling the adjoint's scaling
+        (adjParams[0]); `iterations` Jacobi sweeps (the reference: 1000).
+        m2norm_out (device (nIC,)): the scaled viscosity per cell."""
+        self._check(u_n, (5, self.nIC))
+        self._check(ua, (5, self.nIC))
+        if m2norm_out is not None:
+            self._check(m2norm_out, (self.nIC,))
+        codes = {'uniform': _lib.VISC_UNIFORM, 'abarbanel': _lib.VISC_ABARBANEL, 'turkel': _lib.VISC_TURKEL}
+        if vtype not in codes:
+            raise ValueError('adjoint viscosity type %r (uniform, abarbanel, turkel)' % (vtype,))
+        self._chk(self.lib.adfvm_adjoint_viscosity(self.handle, _dptr(u_n), _dptr(ua), float(dt), codes[vtype],
+                                                   float(scaling), int(iterations), _dptr(m2norm_out),
+                                                   _stream(stream)))
+
     def bc_state(self):
         """Zeroed (5, nBoundaryFaces) device array for adjoint_step's bc_grad."""
         return self.torch.zeros((5, self.nBF), dtype=self.tdtype, device='cuda')

@@ -598,6 +598,7 @@ struct adfvm_ctx {
     int* nbFull = nullptr;       // ghost-extended neighbour [nF]
     void* ghost = nullptr;
     void* sgeo = nullptr;      // [6][4][nIC] cell-slot gradient geometry
+    void* visc = nullptr;      // adjoint viscosity: [6][nIC] slot coefficients, [nIC] diagonal, [nIC] eigenvalue
     void* u1 = nullptr; void* u2 = nullptr;
     void* grad0 = nullptr; void* grad1 = nullptr; void* grad2 = nullptr;
     void* adjAcc = nullptr;    // [5][nIC] adjoint of u0 accumulated over the stages
@@ -1059,7 +1060,7 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
-                    c->segCell, c->segBlock, c->blockSeg, c->pos1, c->c1off, c->sgeo};
+                    c->segCell, c->segBlock, c->blockSeg, c->pos1, c->c1off, c->sgeo, c->visc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -1264,3 +1265,4 @@ extern "C" int adfvm_unpack_state(adfvm_ctx* c, const void* u, void* rho, void* 
 }
 
 #include "adjoint_kernels.inc"
+#include "viscosity.inc"

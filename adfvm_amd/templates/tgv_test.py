@@ -1,7 +1,9 @@
 """Taylor-Green vortex case file: kinetic-energy objective, Gaussian source
 perturbation (the shape of templates/cylinder_test.py's perturb).
 
-Environment: ADFVM_CASE_DIR (default cases/tgv<N>), ADFVM_TGV_N (default 12).
+Environment: ADFVM_CASE_DIR (default cases/tgv<N>), ADFVM_TGV_N (default 12),
+ADFVM_ADJ_VISC (adjoint-viscosity type: uniform, abarbanel, turkel; default
+none) with ADFVM_ADJ_SCALING (its scaling, default 0).
 """
 import os
 
@@ -45,6 +47,7 @@ def perturb(fields, mesh, t):
 
 
 parameters = 'source'
+adjParams = [float(os.environ.get('ADFVM_ADJ_SCALING', '0')), os.environ.get('ADFVM_ADJ_VISC') or None, None]
 
 nSteps = 20
 writeInterval = 10

@@ -250,6 +250,24 @@ int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* s
                        void* ua, double obja, void* source_grad, void* bc_grad, const void* tape,
                        void* stream);
 
+/* Adjoint-viscosity stabilisation of the adjoint state after an adjoint step
+ * (the viscous map of apps/adjoint.py:127-140 with computeAdjointViscosity
+ * and viscositySolver, postpro.py:491-719; the GPU heat-equation solve of
+ * matop_cuda.cpp:108-235).  From the primal state u_n (device [5][nIC], the
+ * step's input) it builds the symmetriser matrix of `type` per cell
+ * (ADFVM_VISC_UNIFORM, _ABARBANEL, _TURKEL; postpro.py:545-627), takes its
+ * largest eigenvalue (postpro.py:629-633), scales it to
+ * eig * scaling / sqrt(sum eig^2 V / sum V) (postpro.py:635-652), and applies
+ * `iterations` Jacobi sweeps of the implicit step
+ * (1 + dt/V sum_f A DT_f/delta_f) phi - dt/V sum_f (A DT_f/delta_f) phi_nb = ua/V
+ * over internal faces (the reference runs 1000), then ua = phi V in place.
+ * m2norm_out (device [nIC], nullable): the scaled viscosity per cell. */
+#define ADFVM_VISC_UNIFORM 0
+#define ADFVM_VISC_ABARBANEL 1
+#define ADFVM_VISC_TURKEL 2
+int adfvm_adjoint_viscosity(adfvm_ctx* ctx, const void* u_n, void* ua, double dt, int32_t type, double scaling,
+                            int32_t iterations, void* m2norm_out, void* stream);
+
 /* Block-size autotuner for the six stencil kernels (gradient, face flux,
  * stage update, face adjoint, adjoint gather, gradient adjoint).  Times each
  * at 64, 128, 256 and 512 threads per block (reps timed launches each) on

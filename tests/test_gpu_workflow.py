@@ -182,6 +182,39 @@ def test_apps_fd_vs_adjoint(tmp_path, monkeypatch):
     assert os.path.exists(os.path.join(case, '0', 'rhoa'))
 
 
+def test_apps_adjoint_viscosity(tmp_path, monkeypatch):
+    """The adjoint app with adjParams = [scaling, 'abarbanel', None] (the
+    viscous map after every reversed step, apps/adjoint.py:127-140,288-289):
+    runs through, its sensitivity and adjoint energies stay finite, and with
+    a small scaling it stays within 5% of the unstabilised sensitivity while
+    differing from it."""
+    from adfvm_amd.apps import problem, adjoint as adjoint_app
+    case = str(tmp_path / 'tgv')
+    monkeypatch.setenv('ADFVM_CASE_DIR', case)
+    monkeypatch.setenv('ADFVM_TGV_N', '10')
+    monkeypatch.chdir(tmp_path)
+    tpl = os.path.join(REPO, 'adfvm_amd', 'templates', 'tgv_test.py')
+    old = config.precision
+    try:
+        problem.main([tpl, 'orig', '--gpu_double'])
+        adjoint_app.main([tpl, '--gpu_double'])
+        monkeypatch.setenv('ADFVM_ADJ_VISC', 'abarbanel')
+        monkeypatch.setenv('ADFVM_ADJ_SCALING', '1e-3')
+        adjoint_app.main([tpl, '--gpu_double'])
+    finally:
+        config.set_precision(old)
+    with open(os.path.join(case, 'objective.txt')) as f:
+        data = f.readlines()
+    print(''.join(data))
+    plain = float(data[1].split(' ')[-2])
+    visc = float(data[2].split(' ')[-2])
+    assert data[2].startswith('adjoint') and data[2].split(' ')[2] == '0.001'
+    assert np.isfinite(visc) and visc != plain
+    assert abs(visc - plain) < 0.05 * abs(plain)
+    energy = np.loadtxt(os.path.join(case, 'energyTimeSeries.txt'))
+    assert np.isfinite(energy).all()
+
+
 def test_apps_cylinder_drag(tmp_path, monkeypatch):
     """The reference's test_adjoint workflow on its own case shape: the
     cylinder with the drag objective (templates/cylinder_test.py: drag on

@@ -448,3 +448,107 @@ def test_step_bc_value_gradient_autograd_vs_central_differences(name):
             errs.append(abs(ad - fd) / abs(fd))
         print(name, field, patch, key, 'rel %.1e %.1e' % tuple(errs))
         assert min(errs) <= 1e-7, (field, patch, key, ad, errs)
+
+
+# ------------------------------------------------- adjoint viscosity --------
+def _visc_linear_box():
+    """Cartesian box (default 'walls' patch), permuted; U = A x + U0 and
+    p = p0 + b.x linear, T uniform, with fixedValue ghosts holding the face
+    values (so face values are exact on every face)."""
+    from oracle import viscosity_oracle as vo
+    md = meshgen.permute_and_rcm(meshgen.box((4, 3, 3), (1.0, 0.9, 0.6)), seed=2)
+    om = mesh_oracle.build(md)
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(3, 3)) * 20
+    U0 = np.array([30., -5., 2.])
+    bvec = rng.normal(size=3) * 500
+    b0 = om.boundary['walls']
+    fc = om.faceCentres[b0['startFace']:b0['startFace'] + b0['nFaces']]
+    bcs = {'walls': {'U': {'type': 'fixedValue', 'value': fc @ A.T + U0},
+                     'T': {'type': 'fixedValue', 'value': np.full((len(fc), 1), 300.)},
+                     'p': {'type': 'fixedValue', 'value': (1e5 + fc @ bvec)[:, None]}}}
+    c = fvm_oracle.Case(om, bcs, G, CP, 0.7, lambda T: T * 0 + 1e-5)
+    x = om.cellCentres[:om.nInternalCells]
+    U = x @ A.T + U0
+    T = np.full((len(x), 1), 300.)
+    p = (1e5 + x @ bvec)[:, None]
+    u = np.concatenate([a.T for a in cons(U, T, p)])
+    return vo, c, om, u, A, bvec
+
+
+def test_viscosity_gradients_exact_for_linear_fields():
+    vo, c, om, u, A, bvec = _visc_linear_box()
+    U, T, p = vo.primitive_with_ghosts(c, u)
+    gU, dU, gp, gc = vo.viscosity_gradients(c, U, T, p)
+    assert np.abs(gU - A[None]).max() < 1e-9 * np.abs(A).max()
+    assert np.abs(dU - np.trace(A)).max() < 1e-9 * np.abs(A).max()
+    assert np.abs(gp - bvec[None]).max() < 1e-9 * np.abs(bvec).max()
+    assert np.abs(gc).max() < 1e-9
+
+
+def test_viscosity_matrix_closed_forms():
+    """The symmetrised matrices against hand-built forms: (1) a uniform state,
+    where only the gradrho term of postpro.py:553 survives: abarbanel's
+    row 0 holds -b*gradrho/rho, turkel's row 4 -(-c^2 gradrho) Uref/pref, and
+    the largest eigenvalue of such an arrow matrix is |r|/2; (2) the linear
+    fields' velocity block, divU/2 I - (gradU + gradU^T)/2."""
+    vo, c, om, u, A, bvec = _visc_linear_box()
+    n = 7
+    Z3 = np.zeros((n, 3))
+    p = np.full((n, 1), 1.1e5)
+    T = np.full((n, 1), 290.)
+    U = np.zeros((n, 3))
+    g = G
+    rho = p[:, 0] / (R * T[:, 0])
+    cc = np.sqrt(g * p[:, 0] / rho)
+    grho = -g * p[:, 0] / cc                                   # (0 - c p) g / c^2
+    for vtype in ('abarbanel', 'turkel'):
+        MS = vo.matrix(c, vtype, U, T, p, np.zeros((n, 3, 3)), np.zeros(n), Z3, Z3)
+        if vtype == 'abarbanel':
+            r = (cc / np.sqrt(g)) * grho / rho                     # b gradrho / rho, each component
+            row, e = 0, np.sqrt(3) * np.abs(r) / 2
+        else:
+            r = (-cc * cc * grho) * vo.UREF / vo.PREF
+            row, e = 4, np.sqrt(3) * np.abs(r) / 2
+        want = np.zeros((n, 5, 5))
+        for k in range(3):
+            want[:, row, 1 + k] = want[:, 1 + k, row] = -r / 2
+        assert np.allclose(MS, want, rtol=1e-13, atol=1e-13 * np.abs(r).max()), vtype
+        assert np.allclose(vo.max_eigenvalue(MS), e, rtol=1e-12), vtype
+    U, T, p = vo.primitive_with_ghosts(c, u)
+    gU, dU, gp, gc = vo.viscosity_gradients(c, U, T, p)
+    MS = vo.matrix(c, 'abarbanel', U, T, p, gU, dU, gp, gc)
+    blk = np.trace(A) / 2 * np.eye(3) - (A + A.T) / 2
+    assert np.abs(MS[:, 1:4, 1:4] - blk[None]).max() < 1e-8 * np.abs(A).max()
+    assert np.array_equal(MS, np.swapaxes(MS, -1, -2))
+
+
+def test_viscosity_heat_step_against_direct_solve():
+    """The Jacobi sweeps converge to the direct solution of the implicit
+    step (I + dt/V L) phi = b over internal faces; a uniform field is
+    unchanged; 'uniform' viscosity is the scaling itself."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    from oracle import viscosity_oracle as vo
+    md = meshgen.permute_and_rcm(meshgen.box((5, 4, 3), (1.0, 0.8, 0.6)), seed=6)
+    om = mesh_oracle.build(md)
+    c = fvm_oracle.Case(om, {}, G, CP, 0.7, lambda T: T * 0 + 1e-5)
+    nIC = om.nInternalCells
+    rng = np.random.default_rng(8)
+    M = 0.5 + rng.random(nIC)
+    nd, cd, nb = vo.heat_coefficients(c, M, 1e-3)
+    rows = np.repeat(np.arange(nIC), 6)
+    keep = nb.reshape(-1) >= 0
+    K = sp.csr_matrix((nd.reshape(-1)[keep], (rows[keep], nb.reshape(-1)[keep])), shape=(nIC, nIC))
+    K = K + sp.diags(cd + 1)
+    b = rng.normal(size=(nIC, 2))
+    direct = np.stack([spl.spsolve(K.tocsc(), b[:, i]) for i in range(2)], 1)
+    phi = vo.jacobi(nd, cd, nb, b, 400)
+    assert np.abs(phi - direct).max() < 1e-12 * np.abs(direct).max()
+    # symmetric coupling: nd_ij V_i == nd_ji V_j
+    KV = (K - sp.diags(K.diagonal())).multiply(om.volumes.reshape(-1, 1))
+    assert abs(KV - KV.T).max() < 1e-15 * abs(KV).max()
+    ones = np.ones((nIC, 1)) * 3.5
+    assert np.allclose(vo.jacobi(nd, cd, nb, ones, 7), 3.5, rtol=1e-14)
+    Mu = vo.scaled_viscosity(c, 'uniform', None, 2.5)
+    assert np.allclose(Mu, 2.5, rtol=1e-15)
