@@ -66,8 +66,8 @@ template <typename T> struct DMesh {
     const int* owner;       // [nF]
     const int* fOther;      // [nF]: neighbour, paired internal cell (cyclic) or -1-b (ghost)
     const unsigned char* fKind;  // [nF]: FACE_INTERIOR / FACE_CHARACTERISTIC / FACE_BOUNDARY
-    const int* pos1;        // [nF]: slot of the face's neighbour-side adjoint record, -1 if none
-    const int* c1off;       // [nIC+1]: first neighbour-side record slot of each cell
+    const int* posP;        // [nF]: record index (plane*nIC + cell) of the face's owner-side adjoint
+    const int* posN;        // [nF]: record index of its neighbour-side adjoint, -1 if none
     const int* bPatch;      // [nBF]
     const DPatch* patches;
     const T* bcval;
@@ -106,8 +106,8 @@ __device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict
 // record is a few 16-byte vector loads instead of one cache line per field:
 //   gradients        [nIC][GREC]   15 used
 //   face fluxes      [nF][FREC]    6 used (F*A, lam*A)
-//   face adjoints    [2][nF][AREC] owner sides by face, then neighbour sides in gather
-//                                  order: gradient adjoint (15), primitive adjoint (5)
+//   face adjoints    [nPlanes][nIC][AREC] one plane per cell slot, then cyclic
+//                                  extras: gradient adjoint (15), primitive adjoint (5)
 //   cell adjoints    [nIC][AREC]   same layout as one side of a face record
 constexpr int GREC = 16, FREC = 8, AREC = 20;
 
@@ -612,9 +612,10 @@ struct adfvm_ctx {
     long* segCell = nullptr; int* segBlock = nullptr; int* blockSeg = nullptr;
     double* scal = nullptr;    // small device scalars
     int* fOther = nullptr; unsigned char* fKind = nullptr;
-    int* pos1 = nullptr; int* c1off = nullptr;
+    int* posP = nullptr; int* posN = nullptr;
+    int nPlanes = 6;           // adjoint record planes: one per cell slot, then cyclic extras
     void* fbuf = nullptr;      // [6][nF] face fluxes * area
-    void* frec = nullptr;      // [2][nF][AREC] face adjoint records (owner sides, neighbour sides in gather order)
+    void* frec = nullptr;      // [nPlanes][nIC][AREC] face adjoint records, by cell slot
     // per-kernel event timing (adfvm_profile)
     bool prof = false;
     std::vector<hipEvent_t> evPool;
@@ -696,7 +697,7 @@ template <typename T> DMesh<T> dmesh(const adfvm_ctx* c) {
     m.sFace = c->sFace; m.sOther = c->sOther; m.sFlags = c->sFlags; m.sPair = c->sPair;
     m.owner = c->owner; m.bPatch = c->bPatch; m.patches = c->patches; m.bcval = (const T*)c->bcval;
     m.fOther = c->fOther; m.fKind = c->fKind;
-    m.pos1 = c->pos1; m.c1off = c->c1off;
+    m.posP = c->posP; m.posN = c->posN;
     return m;
 }
 
@@ -774,7 +775,7 @@ template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* 
     HIPCHK(ctx, hipMalloc(&ctx->adjAcc, st));
     HIPCHK(ctx, hipMalloc(&ctx->adjG, (size_t)AREC * nIC * sizeof(T)));
     HIPCHK(ctx, hipMalloc(&ctx->fbuf, (size_t)FREC * nF * sizeof(T)));
-    HIPCHK(ctx, hipMalloc(&ctx->frec, (size_t)2 * AREC * nF * sizeof(T)));
+    HIPCHK(ctx, hipMalloc(&ctx->frec, (size_t)AREC * ctx->nPlanes * nIC * sizeof(T)));
     return ADFVM_OK;
 }
 
@@ -1007,33 +1008,45 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
             fKind[f] = Pd.fluxKind == ADFVM_FLUX_CHARACTERISTIC ? FACE_CHARACTERISTIC : FACE_BOUNDARY;
         }
     }
-    // Neighbour-side face adjoint records are stored in the order the cells
-    // gather them: cell cc's records, in its slot order, at c1off[cc] ...
-    // (a face's neighbour side is needed by exactly one cell: the internal
-    // neighbour, or the partner of a cyclic face).
-    std::vector<int> pos1(nF, -1), c1off(nIC + 1, 0);
-    {
-        int k = 0;
-        for (long cc = 0; cc < nIC; cc++) {
-            c1off[cc] = k;
-            for (int j = 0; j < 6; j++) {
-                const int fl = sFlags[j * nIC + cc];
-                const int kind = fl >> 1;
-                int g = -1;
-                if (kind == SLOT_INTERNAL && !(fl & 1)) g = sFace[j * nIC + cc];
-                else if (kind == SLOT_COUPLED) g = sPair[j * nIC + cc];
-                if (g < 0) continue;
-                if (pos1[g] != -1) return bad("face with two neighbour-side slots");
-                pos1[g] = k++;
-            }
+    // Face adjoint records sit in planes of nIC records: plane j holds, for
+    // every cell, the record of its slot j (its own side of an owned face,
+    // the neighbour side of a face it neighbours); planes 6, 7, ... hold, in
+    // slot order, the neighbour sides of the paired faces of a cell's cyclic
+    // slots (on which the cell is the ghost).  A cell's records for one slot
+    // are then at the same index of every plane, so neighbouring threads of
+    // the gather read neighbouring records.
+    std::vector<int> posP(nF, -1), posN(nF, -1);
+    int maxExtra = 0;
+    for (long cc = 0; cc < nIC; cc++) {
+        int e = 0;
+        for (int j = 0; j < 6; j++) {
+            const int fl = sFlags[j * nIC + cc];
+            const int f = sFace[j * nIC + cc];
+            int& slot = (fl & 1) ? posP[f] : posN[f];
+            if (slot != -1) return bad("face in two slots of the same side");
+            slot = (int)(j * nIC + cc);
+            if ((fl >> 1) == SLOT_COUPLED) e++;
         }
-        c1off[nIC] = k;
-        for (long f = 0; f < nF; f++)
-            if (fKind[f] == FACE_INTERIOR && pos1[f] < 0) return bad("face without a neighbour-side slot");
+        maxExtra = std::max(maxExtra, e);
     }
+    if ((long)(6 + maxExtra) * nIC > 0x7fffffffL) return bad("mesh too large for 32-bit record indices");
+    for (long cc = 0; cc < nIC; cc++) {
+        int e = 0;
+        for (int j = 0; j < 6; j++) {
+            if ((sFlags[j * nIC + cc] >> 1) != SLOT_COUPLED) continue;
+            const int g = sPair[j * nIC + cc];
+            if (posN[g] != -1) return bad("face with two neighbour-side slots");
+            posN[g] = (int)((6 + e++) * nIC + cc);
+        }
+    }
+    for (long f = 0; f < nF; f++) {
+        if (posP[f] < 0) return bad("face without an owner slot");
+        if (fKind[f] == FACE_INTERIOR && posN[f] < 0) return bad("face without a neighbour-side slot");
+    }
+    c->nPlanes = 6 + maxExtra;
     int rc;
-    if ((rc = upload(c, (void**)&c->pos1, pos1)) != ADFVM_OK) return rc;
-    if ((rc = upload(c, (void**)&c->c1off, c1off)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->posP, posP)) != ADFVM_OK) return rc;
+    if ((rc = upload(c, (void**)&c->posN, posN)) != ADFVM_OK) return rc;
     std::vector<int> owner(d->owner, d->owner + nF), nbFull(d->neighbour, d->neighbour + nF);
     if ((rc = upload(c, (void**)&c->fOther, fOther)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->fKind, fKind)) != ADFVM_OK) return rc;
@@ -1060,7 +1073,7 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
-                    c->segCell, c->segBlock, c->blockSeg, c->pos1, c->c1off, c->sgeo, c->visc};
+                    c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->visc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
