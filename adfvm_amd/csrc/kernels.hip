@@ -57,6 +57,8 @@ template <typename T> struct DMesh {
     int nIC, nIF, nF, nBF;
     const T* geo;           // [G_ROWS][nF]
     const T* vol;           // [nIC]
+    const T* ngeo;          // [6][3][nIC]: per cell slot, the neighbour's gradient stencil seen from
+                            // this cell: coef * S*n / V_other (zero on ghost slots)
     const T* sgeo;          // [6][4][nIC]: per cell slot, S*n signed outward for the cell (3) and
                             // the weight of the other side's value in the face value (op.py:47-58)
     const int* sFace;       // [6][nIC]
@@ -590,6 +592,7 @@ struct adfvm_ctx {
     Phys<double> phd;
     std::string err;
     std::vector<DPatch> hpatches;
+    std::vector<int> hBPatch;  // boundary face -> patch (host copy)
     // device
     void* geo = nullptr; void* vol = nullptr; void* bcval = nullptr;
     int* sFace = nullptr; int* sOther = nullptr; unsigned char* sFlags = nullptr; int* sPair = nullptr;
@@ -598,6 +601,7 @@ struct adfvm_ctx {
     int* nbFull = nullptr;       // ghost-extended neighbour [nF]
     void* ghost = nullptr;
     void* sgeo = nullptr;      // [6][4][nIC] cell-slot gradient geometry
+    void* ngeo = nullptr;      // [6][3][nIC] cell-slot transposed-stencil coefficients
     void* visc = nullptr;      // adjoint viscosity: [6][nIC] slot coefficients, [nIC] diagonal, [nIC] eigenvalue
     void* u1 = nullptr; void* u2 = nullptr;
     void* grad0 = nullptr; void* grad1 = nullptr; void* grad2 = nullptr;
@@ -693,7 +697,7 @@ template <typename T> Phys<T> phys_cast(const Phys<double>& p) {
 template <typename T> DMesh<T> dmesh(const adfvm_ctx* c) {
     DMesh<T> m;
     m.nIC = c->nIC; m.nIF = c->nIF; m.nF = c->nF; m.nBF = c->nBF;
-    m.geo = (const T*)c->geo; m.vol = (const T*)c->vol; m.sgeo = (const T*)c->sgeo;
+    m.geo = (const T*)c->geo; m.vol = (const T*)c->vol; m.sgeo = (const T*)c->sgeo; m.ngeo = (const T*)c->ngeo;
     m.sFace = c->sFace; m.sOther = c->sOther; m.sFlags = c->sFlags; m.sPair = c->sPair;
     m.owner = c->owner; m.bPatch = c->bPatch; m.patches = c->patches; m.bcval = (const T*)c->bcval;
     m.fOther = c->fOther; m.fKind = c->fKind;
@@ -743,6 +747,35 @@ template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* 
             sg[(size_t)(4 * j + 3) * nIC + c] = (T)(own ? 1.0 - w : w);
         }
     if ((rc = upload(ctx, &ctx->sgeo, sg)) != ADFVM_OK) return rc;
+    // The transposed gradient stencil for the gradient adjoint: cell c's
+    // slot j face value enters the gradient of the other side o with weight
+    // coef = (1 - w') and outward vector S*n_o, so d(grad_o)/d(w_c) through
+    // that face is coef * S*n_o / V_o:
+    //   internal face: S*n_o = -(S*n_c), coef = -w_f (c owns f) or 1 - w_f
+    //   cyclic face:   the paired face fp seen from its owner o: S*n_fp and
+    //                  coef = 1 - w_fp.
+    std::vector<T> ng((size_t)18 * nIC, T(0));
+    for (long c = 0; c < nIC; c++)
+        for (int j = 0; j < 6; j++) {
+            const long f = d->cellFaces[6 * c + j];
+            const bool own = d->cellOwner[6 * c + j] != 0;
+            long g = -1, o = -1;
+            double coef = 0.0;
+            if (f < ctx->nIF) {
+                g = f;
+                o = own ? d->neighbour[f] : d->owner[f];
+                coef = own ? -d->weights[f] : 1.0 - d->weights[f];
+            } else {
+                const adfvm_patch_desc& Pd = d->patches[ctx->hBPatch[f - ctx->nIF]];
+                if (Pd.fluxKind != ADFVM_FLUX_COUPLED) continue;
+                g = d->patches[Pd.neighbourPatch].startFace + (f - Pd.startFace);
+                o = d->owner[g];
+                coef = 1.0 - d->weights[g];
+            }
+            for (int k = 0; k < 3; k++)
+                ng[(size_t)(3 * j + k) * nIC + c] = (T)(coef * (d->areas[g] * d->normals[3 * g + k]) / d->volumes[o]);
+        }
+    if ((rc = upload(ctx, &ctx->ngeo, ng)) != ADFVM_OK) return rc;
     // boundary-condition values
     std::vector<T> vals;
     for (int p = 0; p < d->nPatches; p++) {
@@ -1058,6 +1091,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     if ((rc = upload(c, (void**)&c->owner, owner)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->nbFull, nbFull)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->bPatch, bPatch)) != ADFVM_OK) return rc;
+    c->hBPatch = bPatch;
     rc = c->prec == 4 ? create_typed<float>(c, d) : create_typed<double>(c, d);
     if (rc != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->patches, c->hpatches)) != ADFVM_OK) return rc;
@@ -1073,7 +1107,7 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
-                    c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->visc};
+                    c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->ngeo, c->visc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
