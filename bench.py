@@ -36,19 +36,20 @@ def kernel_bytes(name, s, nC, nF, nG):
     needs counted once (s = scalar bytes).  Cell slot tables: 6 x (int32
     face, int32 neighbour, uint8 flags) = 54 B per cell, or 30 B without the
     neighbour column; face tables: int32 owner + int32 neighbour + uint8 kind
-    = 9 B per face."""
+    = 9 B per face; the two record indices of a face's adjoint (int32) 8 B
+    per face."""
     if name == 'grad_kernel':          # u, V, face (A, n, w) -> 15 gradients
         return nC * (5 + 1 + 15) * s + nF * 5 * s + 54 * nC
     if name == 'face_flux_kernel':     # u, grad, 17 face rows -> 6 face values
         return nC * (5 + 15) * s + nF * (17 + 6) * s + 9 * nF
     if name == 'stage_kernel':         # face values, V, u_i, u0 (2 of 3 stages) -> u_{i+1}
         return nF * 6 * s + nC * (1 + 5 + 10. / 3 + 5) * s + 30 * nC
-    if name == 'face_vjp_kernel':      # u, grad, ubar, V, 17 face rows -> 40 face adjoint values
-        return nC * (5 + 15 + 5 + 1) * s + nF * (17 + 40) * s + 9 * nF
-    if name == 'vjp_gather_kernel':    # 40 face adjoint values -> wbar, gbar
-        return nF * 40 * s + nC * 20 * s + 30 * nC
-    if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, face (A,n,w) -> ubar, acc
-        return nC * (5 + 5 + 15 + 5 + 10. / 3 + 1 + 5 + 10. / 3) * s + nF * 5 * s + 54 * nC
+    if name == 'face_vjp_kernel':      # u, grad, ubar, V, 17 face rows -> 40 face adjoint values at 2 record indices
+        return nC * (5 + 15 + 5 + 1) * s + nF * (17 + 40) * s + 9 * nF + 8 * nF
+    if name == 'vjp_gather_kernel':    # 40 face adjoint values -> wbar, gbar; slot flags
+        return nF * 40 * s + nC * 20 * s + 6 * nC
+    if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, 6 x (4 + 3) slot coefficients -> ubar, acc
+        return nC * (5 + 5 + 15 + 5 + 10. / 3 + 1 + 5 + 10. / 3 + 42) * s + 30 * nC
     return None
 
 
