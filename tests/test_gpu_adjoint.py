@@ -274,3 +274,60 @@ def test_bc_value_gradient_vs_central_fd_multistep():
         rel = abs(adj - fd) / abs(fd)
         print('bc fd %s/%s/%s: adjoint %.12e fd %.12e rel %.2e' % (field, patch, key, adj, fd, rel))
         assert rel <= 1e-6, (field, patch, key, rel)
+
+
+def _degenerate_cases():
+    """Edge cases of the index tables: periodic boxes of 1 cell (all six
+    faces cyclic, the cell its own partner: six extra record planes) and 2
+    cells per direction (each neighbour reached once through an internal
+    face and once through a cyclic one); a mixed box with an empty (zero-face)
+    patch listed first."""
+    from collections import OrderedDict
+    from adfvm_amd import meshgen
+    out = {}
+    for n in (1, 2):
+        md = meshgen.periodic_box(n)
+        case = cf.CaseDef(md, {}, 1e-3, objective=('kinetic_energy',))
+        rng = np.random.default_rng(n)
+        nIC = n ** 3
+        U = rng.normal(size=(nIC, 3)) * 30
+        T = 300 + rng.random((nIC, 1))
+        p = 1e5 + 100 * rng.random((nIC, 1))
+        out['periodic%d' % n] = (case, cf.state_from_prims(U, T, p))
+    base = cf.mixed_box(1, 'CBC_UPT')
+    md = base.md
+    nF = len(md.owner)
+    b = OrderedDict([('aaa_empty', dict(type='patch', startFace=nF, nFaces=0))])
+    b.update(md.boundary)
+    md2 = meshgen.MeshData(md.points, md.faces, md.owner, md.neighbour, b)
+    bcs = dict(base.bcs)
+    bcs['aaa_empty'] = {'U': {'type': 'fixedValue', 'value': np.zeros((0, 3))},
+                        'T': {'type': 'zeroGradient'}, 'p': {'type': 'zeroGradient'}}
+    case = cf.CaseDef(md2, bcs, base.mu0)
+    out['mixed_empty_patch'] = (case, cf.smooth_state(case, 1))
+    return out
+
+
+@pytest.mark.parametrize('name', ['periodic1', 'periodic2', 'mixed_empty_patch'])
+def test_degenerate_meshes_step_and_adjoint(name):
+    from tests.test_gpu_primal import rel_l2 as rl2
+    case, u = _degenerate_cases()[name]
+    dt = 1e-6
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64)
+    oc = case.oracle()
+    gi = [t(a) for a in cf.ghost_init(case)]
+    r, rU, rE, _, _ = oc.step(t(u[0:1].T), t(u[1:4].T), t(u[4:5].T), dt, ghostInit=gi)
+    u_o = np.concatenate([r.T.numpy(), rU.T.numpy(), rE.T.numpy()])
+    ua = scaled_rbar(case, u, 8) * 1e3
+    obja = 1.0 if case.objective else 0.0
+    (ref_u,) = oracle_step_vjp(case, u, dt, ua, obja)
+    ctx = case.context(np.float64)
+    ud = torch.as_tensor(u, device='cuda').contiguous()
+    u1 = ud.clone()
+    ctx.primal_step(u1, dt)
+    e_step = rl2(u1.cpu().numpy(), u_o)
+    uad = torch.as_tensor(ua, device='cuda').contiguous()
+    ctx.adjoint_step(ud, dt, uad, obja=obja)
+    e_adj = rl2(uad.cpu().numpy(), ref_u)
+    print('%s: step rel-L2 %.2e, adjoint step rel-L2 %.2e' % (name, e_step, e_adj))
+    assert e_step < 1e-12 and e_adj < 1e-11
