@@ -40,6 +40,19 @@ __device__ __forceinline__ int xcd_block() {
     return (int)(x * q + (x < r ? x : r) + k);
 }
 
+// The face kernels read both cells of a face; there a contiguous eighth per
+// XCD measured slower (though it cut their HBM reads by a fifth), while
+// chunks of 16 consecutive blocks per XCD, the chunks dealt round-robin,
+// measured 1-2% faster than the plain order.  Bijection of [0, gridDim.x)
+// (the last partial chunk keeps the plain order).
+__device__ __forceinline__ int xcd_chunk_block() {
+    const unsigned nb = gridDim.x, b = blockIdx.x, C = 16;
+    const unsigned full = nb / (8u * C) * (8u * C);
+    if (b >= full) return (int)b;
+    const unsigned chunk = b / (8u * C), r = b % (8u * C);
+    return (int)(chunk * 8u * C + (r & 7u) * C + (r >> 3));
+}
+
 // ------------------------------------------------------------- tables ----
 // slot flags: bit 0 = this cell owns the face; bits 1-2 = slot kind
 enum { SLOT_INTERNAL = 0, SLOT_COUPLED = 1, SLOT_CHARACTERISTIC = 2, SLOT_BOUNDARY = 3 };
@@ -278,7 +291,7 @@ template <typename T, int BS>
 __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                           const T* __restrict__ grad, const T* __restrict__ ghost,
                                                           T* __restrict__ fbuf) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_chunk_block() * BS + threadIdx.x;
     if (f >= m.nF) return;
     const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
     const int P = m.owner[f];
