@@ -131,6 +131,15 @@ class Case(object):
                 gsrc[p['cstart']:p['cstart'] + p['n']] = self.owner[p['start']:p['start'] + p['n']]
         self.grad_src = gsrc
 
+    def to(self, device):
+        """Move the case's tensors to `device` (torch runs the same
+        restatement there; inputs must be on that device too).  Returns
+        self."""
+        self.t = {k: v.to(device) for k, v in self.t.items()}
+        for k in ('owner', 'neighbour', 'cellFaces', 'cellOwner', 'cellNeighbours', 'grad_src'):
+            setattr(self, k, getattr(self, k).to(device))
+        return self
+
     # --------------------------------------------------------- primitives --
     def primitive(self, rho, rhoU, rhoE):
         """adFVM/density.py:162-171."""
@@ -156,8 +165,9 @@ class Case(object):
         BCs.py:152-184).  U,T,p: internal (nIC,k); returns (nC,k)."""
         nG = self.nC - self.nIC
         if ghostInit is None:
-            ghostInit = [torch.zeros(nG, 3, dtype=U.dtype), torch.zeros(nG, 1, dtype=U.dtype),
-                         torch.zeros(nG, 1, dtype=U.dtype)]
+            ghostInit = [torch.zeros(nG, 3, dtype=U.dtype, device=U.device),
+                         torch.zeros(nG, 1, dtype=U.dtype, device=U.device),
+                         torch.zeros(nG, 1, dtype=U.dtype, device=U.device)]
         fields = {'U': torch.cat([U, ghostInit[0]]), 'T': torch.cat([T, ghostInit[1]]),
                   'p': torch.cat([p, ghostInit[2]])}
         n_all = self.t['normals']
@@ -183,7 +193,7 @@ class Case(object):
                         nn = n_all[s:s + n]
                         val = val - (val * nn).sum(-1, keepdim=True) * nn
                 elif kind == 'fixedValue':
-                    val = bvalue(pt['spec'][name]['value'], n, 3 if name == 'U' else 1).to(phi.dtype) * 1
+                    val = bvalue(pt['spec'][name]['value'], n, 3 if name == 'U' else 1).to(phi) * 1
                 elif kind in ('calculated', 'CBC_UPT', 'CBC_TOTAL_PT'):
                     continue
                 else:
@@ -205,15 +215,15 @@ class Case(object):
             s, n, cs = pt['start'], pt['n'], pt['cstart']
             sp = pt['spec']['p']
             if pt['cbc'] == 'CBC_UPT':
-                Ub = bvalue(sp['U0'], n, 3) * 1
-                Tb = bvalue(sp['T0'], n, 1) * 1
-                pb = bvalue(sp['p0'], n, 1) * 1
+                Ub = bvalue(sp['U0'], n, 3).to(fields['U']) * 1
+                Tb = bvalue(sp['T0'], n, 1).to(fields['U']) * 1
+                pb = bvalue(sp['p0'], n, 1).to(fields['U']) * 1
             else:
                 own = self.owner[s:s + n]
                 Uo, To = fields['U'][own], fields['T'][own]
-                d = bvalue(sp['direction'], n, 3) if 'direction' in sp else n_all[s:s + n]
-                Tt = bvalue(sp['Tt'], n, 1)
-                ptot = bvalue(sp['pt'], n, 1)
+                d = bvalue(sp['direction'], n, 3).to(Uo) if 'direction' in sp else n_all[s:s + n]
+                Tt = bvalue(sp['Tt'], n, 1).to(Uo)
+                ptot = bvalue(sp['pt'], n, 1).to(Uo)
                 Un = (Uo * d).sum(-1, keepdim=True)
                 Ub = Un * d
                 Tb = Tt - 0.5 * Un * Un / self.Cp
@@ -450,7 +460,7 @@ class Case(object):
     # ---------------------------------------------------------- objectives --
     def objective_value(self, rho, rhoU, rhoE, U, T, p):
         if self.objective is None:
-            return torch.zeros((), dtype=rho.dtype)
+            return torch.zeros((), dtype=rho.dtype, device=rho.device)
         kind = self.objective[0]
         if kind == 'kinetic_energy':
             Uc = rhoU / rho
