@@ -587,12 +587,6 @@ __global__ void __launch_bounds__(BLOCK) copy_kernel(long n, const T* __restrict
     if (i < n) y[i] = x[i];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) axpby_kernel(long n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
-    long i = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) y[i] = a * x[i] + b * y[i];
-}
-
 // ----------------------------------------------------------- context -----
 
 struct adfvm_ctx {
