@@ -70,8 +70,8 @@ template <typename T> struct DMesh {
     int nIC, nIF, nF, nBF;
     const T* geo;           // [G_ROWS][nF]
     const T* vol;           // [nIC]
-    const T* ngeo;          // [6][3][nIC]: per cell slot, the neighbour's gradient stencil seen from
-                            // this cell: coef * S*n / V_other (zero on ghost slots)
+    const T* ngeo;          // [6][3][nIC]: per coupled cell slot, the paired face's owner's gradient
+                            // stencil seen from this cell: coef * S*n / V_other (zero elsewhere)
     const T* sgeo;          // [6][4][nIC]: per cell slot, S*n signed outward for the cell (3) and
                             // the weight of the other side's value in the face value (op.py:47-58)
     const int* sFace;       // [6][nIC]
@@ -757,28 +757,21 @@ template <typename T> int create_typed(adfvm_ctx* ctx, const adfvm_solver_desc* 
     // The transposed gradient stencil for the gradient adjoint: cell c's
     // slot j face value enters the gradient of the other side o with weight
     // coef = (1 - w') and outward vector S*n_o, so d(grad_o)/d(w_c) through
-    // that face is coef * S*n_o / V_o:
-    //   internal face: S*n_o = -(S*n_c), coef = -w_f (c owns f) or 1 - w_f
-    //   cyclic face:   the paired face fp seen from its owner o: S*n_fp and
-    //                  coef = 1 - w_fp.
+    // that face is coef * S*n_o / V_o.  Across an internal face that is
+    // -(1 - w') S*n_c / V_o (w' the other side's weight), which the kernel
+    // forms from the slot geometry; across a cyclic face it is the paired
+    // face fp seen from its owner o: S*n_fp and coef = 1 - w_fp, tabulated
+    // here.
     std::vector<T> ng((size_t)18 * nIC, T(0));
     for (long c = 0; c < nIC; c++)
         for (int j = 0; j < 6; j++) {
             const long f = d->cellFaces[6 * c + j];
-            const bool own = d->cellOwner[6 * c + j] != 0;
-            long g = -1, o = -1;
-            double coef = 0.0;
-            if (f < ctx->nIF) {
-                g = f;
-                o = own ? d->neighbour[f] : d->owner[f];
-                coef = own ? -d->weights[f] : 1.0 - d->weights[f];
-            } else {
-                const adfvm_patch_desc& Pd = d->patches[ctx->hBPatch[f - ctx->nIF]];
-                if (Pd.fluxKind != ADFVM_FLUX_COUPLED) continue;
-                g = d->patches[Pd.neighbourPatch].startFace + (f - Pd.startFace);
-                o = d->owner[g];
-                coef = 1.0 - d->weights[g];
-            }
+            if (f < ctx->nIF) continue;
+            const adfvm_patch_desc& Pd = d->patches[ctx->hBPatch[f - ctx->nIF]];
+            if (Pd.fluxKind != ADFVM_FLUX_COUPLED) continue;
+            const long g = d->patches[Pd.neighbourPatch].startFace + (f - Pd.startFace);
+            const long o = d->owner[g];
+            const double coef = 1.0 - d->weights[g];
             for (int k = 0; k < 3; k++)
                 ng[(size_t)(3 * j + k) * nIC + c] = (T)(coef * (d->areas[g] * d->normals[3 * g + k]) / d->volumes[o]);
         }

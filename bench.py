@@ -48,8 +48,8 @@ def kernel_bytes(name, s, nC, nF, nG):
         return nC * (5 + 15 + 5 + 1) * s + nF * (17 + 40) * s + 9 * nF + 8 * nF
     if name == 'vjp_gather_kernel':    # 40 face adjoint values -> wbar, gbar; slot flags
         return nF * 40 * s + nC * 20 * s + 6 * nC
-    if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, 6 x (4 + 3) slot coefficients -> ubar, acc
-        return nC * (5 + 5 + 15 + 5 + 10. / 3 + 1 + 5 + 10. / 3 + 42) * s + 30 * nC
+    if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, 6 x 4 slot coefficients -> ubar, acc
+        return nC * (5 + 5 + 15 + 5 + 10. / 3 + 1 + 5 + 10. / 3 + 24) * s + 30 * nC
     return None
 
 
