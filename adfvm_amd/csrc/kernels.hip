@@ -83,6 +83,7 @@ template <typename T> struct DMesh {
     const unsigned char* fKind;  // [nF]: FACE_INTERIOR / FACE_CHARACTERISTIC / FACE_BOUNDARY
     const int* posP;        // [nF]: record index (plane*nIC + cell) of the face's owner-side adjoint
     const int* posN;        // [nF]: record index of its neighbour-side adjoint, -1 if none
+    int ownerSum;           // the face adjoint sums each cell's owned internal faces into plane 0
     const int* bPatch;      // [nBF]
     const DPatch* patches;
     const T* bcval;
@@ -625,6 +626,11 @@ struct adfvm_ctx {
     int* fOther = nullptr; unsigned char* fKind = nullptr;
     int* posP = nullptr; int* posN = nullptr;
     int nPlanes = 6;           // adjoint record planes: one per cell slot, then cyclic extras
+    // internal faces sorted by owner: per block size, the face ranges of the
+    // owner-summing face adjoint blocks (no owner split between two ranges)
+    bool ownerSum = false;
+    int* fvjpBlk[4] = {nullptr, nullptr, nullptr, nullptr};
+    int fvjpNBlk[4] = {0, 0, 0, 0};
     void* fbuf = nullptr;      // [6][nF] face fluxes * area
     void* frec = nullptr;      // [nPlanes][nIC][AREC] face adjoint records, by cell slot
     // per-kernel event timing (adfvm_profile)
@@ -682,6 +688,7 @@ enum { K_GRAD = 0, K_FLUX, K_STAGE, K_FVJP, K_GATHER, K_GVJP, K_TUNABLE };
 const char* const kTunableName[K_TUNABLE] = {"grad_kernel", "face_flux_kernel", "stage_kernel",
                                               "face_vjp_kernel", "vjp_gather_kernel", "grad_vjp_kernel"};
 const int kBlockChoices[4] = {64, 128, 256, 512};
+int block_choice(int bs) { return bs == 64 ? 0 : bs == 128 ? 1 : bs == 512 ? 3 : 2; }
 
 // Calls f(std::integral_constant<int, BS>) for the block size bs.
 template <typename F> void with_block(int bs, F&& f) {
@@ -709,6 +716,7 @@ template <typename T> DMesh<T> dmesh(const adfvm_ctx* c) {
     m.owner = c->owner; m.bPatch = c->bPatch; m.patches = c->patches; m.bcval = (const T*)c->bcval;
     m.fOther = c->fOther; m.fKind = c->fKind;
     m.posP = c->posP; m.posN = c->posN;
+    m.ownerSum = c->ownerSum ? 1 : 0;
     return m;
 }
 
@@ -1078,6 +1086,38 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     }
     c->nPlanes = 6 + maxExtra;
     int rc;
+    // The face adjoint sums a cell's owner sides in LDS when one owner's
+    // internal faces are consecutive (owners non-decreasing over the internal
+    // faces) and are the cell's first slots, in face order: the sum is then
+    // the gather's own sum over those slots.  Per block size, the face ranges
+    // end at owner changes.
+    c->ownerSum = true;
+    for (long f = 1; f < nIF && c->ownerSum; f++)
+        if (d->owner[f - 1] > d->owner[f]) c->ownerSum = false;
+    for (long cc = 0; cc < nIC && c->ownerSum; cc++) {
+        int prev = -1;
+        bool more = true;              // still in the leading run of owned internal slots
+        for (int j = 0; j < 6; j++) {
+            const int fl = sFlags[j * nIC + cc];
+            const bool ownInt = (fl & 1) && (fl >> 1) == SLOT_INTERNAL;
+            if (ownInt && (!more || sFace[j * nIC + cc] <= prev)) { c->ownerSum = false; break; }
+            if (ownInt) prev = sFace[j * nIC + cc];
+            else more = false;
+        }
+    }
+    for (int k = 0; k < 4 && c->ownerSum; k++) {
+        const long bs = kBlockChoices[k];
+        std::vector<int> starts(1, 0);
+        for (long f = 0; f < nIF;) {
+            long e = std::min(f + bs, nIF);
+            while (e < nIF && e > f && d->owner[e - 1] == d->owner[e]) e--;
+            if (e == f) return bad("cell with more internal faces than a face adjoint block");
+            starts.push_back((int)e);
+            f = e;
+        }
+        c->fvjpNBlk[k] = (int)starts.size() - 1;
+        if ((rc = upload(c, (void**)&c->fvjpBlk[k], starts)) != ADFVM_OK) return rc;
+    }
     if ((rc = upload(c, (void**)&c->posP, posP)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->posN, posN)) != ADFVM_OK) return rc;
     std::vector<int> owner(d->owner, d->owner + nF), nbFull(d->neighbour, d->neighbour + nF);
@@ -1107,7 +1147,8 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->patches, c->cellNbFull, c->nbFull, c->ghost, c->u1, c->u2, c->grad0, c->grad1,
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
-                    c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->ngeo, c->visc};
+                    c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->ngeo, c->visc,
+                    c->fvjpBlk[0], c->fvjpBlk[1], c->fvjpBlk[2], c->fvjpBlk[3]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }

@@ -31,23 +31,28 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 
 
-def kernel_bytes(name, s, nC, nF, nG):
+def kernel_bytes(name, s, nC, nF, nG, nO=None):
     """Algorithmic bytes one launch moves: each array element the kernel
     needs counted once (s = scalar bytes).  Cell slot tables: 6 x (int32
     face, int32 neighbour, uint8 flags) = 54 B per cell, or 30 B without the
     neighbour column; face tables: int32 owner + int32 neighbour + uint8 kind
-    = 9 B per face; the two record indices of a face's adjoint (int32) 8 B
-    per face."""
+    = 9 B per face; the record indices of a face's adjoint (int32) 4 B per
+    record written by face.  Face adjoint records (20 scalars): with nO cells
+    owning internal faces (owner sides summed per cell) one per internal
+    face, one per such cell and two per cyclic face; otherwise two per face
+    (nO None).  Every boundary face here is cyclic (periodic box)."""
     if name == 'grad_kernel':          # u, V, face (A, n, w) -> 15 gradients
         return nC * (5 + 1 + 15) * s + nF * 5 * s + 54 * nC
     if name == 'face_flux_kernel':     # u, grad, 17 face rows -> 6 face values
         return nC * (5 + 15) * s + nF * (17 + 6) * s + 9 * nF
     if name == 'stage_kernel':         # face values, V, u_i, u0 (2 of 3 stages) -> u_{i+1}
         return nF * 6 * s + nC * (1 + 5 + 10. / 3 + 5) * s + 30 * nC
-    if name == 'face_vjp_kernel':      # u, grad, ubar, V, 17 face rows -> 40 face adjoint values at 2 record indices
-        return nC * (5 + 15 + 5 + 1) * s + nF * (17 + 40) * s + 9 * nF + 8 * nF
-    if name == 'vjp_gather_kernel':    # 40 face adjoint values -> wbar, gbar; slot flags
-        return nF * 40 * s + nC * 20 * s + 6 * nC
+    recs = 2 * nF if nO is None else (nF - nG) + nO + 2 * nG
+    idx = 8 * nF if nO is None else 4 * (nF - nG) + 8 * nG
+    if name == 'face_vjp_kernel':      # u, grad, ubar, V, 17 face rows -> face adjoint records
+        return nC * (5 + 15 + 5 + 1) * s + nF * 17 * s + recs * 20 * s + 9 * nF + idx
+    if name == 'vjp_gather_kernel':    # face adjoint records -> wbar, gbar; slot flags
+        return recs * 20 * s + nC * 20 * s + 6 * nC
     if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, 6 x 4 slot coefficients -> ubar, acc
         return nC * (5 + 5 + 15 + 5 + 10. / 3 + 1 + 5 + 10. / 3 + 24) * s + 30 * nC
     return None
@@ -281,6 +286,9 @@ def main():
     m = case.mesh
     nC, nF = m.nInternalCells, m.nFaces
     nG = m.nFaces - m.nInternalFaces
+    own = np.asarray(m.owner[:m.nInternalFaces])
+    # the face adjoint sums owner sides per cell when internal faces are in owner order
+    nO = len(np.unique(own)) if np.all(np.diff(own) >= 0) else None
     u0 = cf.tgv_state(case, params)
     dx = 2 * np.pi / args.n
     c0 = np.sqrt(cf.GAMMA * cf.R * params['T0'])
@@ -309,7 +317,7 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     ms_tot, launches = prof[dom]
     avg_s = ms_tot / launches * 1e-3
-    kb = kernel_bytes(dom, 4, nC, nF, nG)
+    kb = kernel_bytes(dom, 4, nC, nF, nG, nO)
     achieved = kb / avg_s / 1e9 if kb else None
     traffic = None
     tf = os.path.join(ROOT, 'profiles', 'traffic.json')

@@ -305,10 +305,25 @@ def _degenerate_cases():
                         'T': {'type': 'zeroGradient'}, 'p': {'type': 'zeroGradient'}}
     case = cf.CaseDef(md2, bcs, base.mu0)
     out['mixed_empty_patch'] = (case, cf.smooth_state(case, 1))
+    # internal faces in a random order, owners no longer non-decreasing: the
+    # face adjoint cannot sum a cell's owner sides in one block and writes a
+    # record per face side (the path of meshes not in owner order)
+    for name, c0, u0 in (('unsorted_mixed', base, cf.smooth_state(base, 2)),
+                         ('unsorted_periodic',) + out['periodic2']):
+        m0 = c0.md
+        nIF = len(m0.neighbour)
+        order = np.random.default_rng(4).permutation(nIF)
+        idx = np.concatenate([order, np.arange(nIF, len(m0.owner))])
+        md3 = meshgen.MeshData(m0.points, m0.faces[idx], m0.owner[idx], m0.neighbour[order], m0.boundary,
+                               m0.nCells)
+        assert np.any(np.diff(md3.owner[:nIF]) < 0)
+        c3 = cf.CaseDef(md3, c0.bcs, c0.mu0, objective=c0.objective)
+        out[name] = (c3, u0)
     return out
 
 
-@pytest.mark.parametrize('name', ['periodic1', 'periodic2', 'mixed_empty_patch'])
+@pytest.mark.parametrize('name', ['periodic1', 'periodic2', 'mixed_empty_patch', 'unsorted_mixed',
+                                  'unsorted_periodic'])
 def test_degenerate_meshes_step_and_adjoint(name):
     from tests.test_gpu_primal import rel_l2 as rl2
     case, u = _degenerate_cases()[name]
