@@ -341,7 +341,8 @@ def main():
         'config': {'workload': 'TGV %d^3 periodic hex, random permutation + RCM, primal+adjoint, SSP-RK3, Roe, '
                                'fixed dt, start states and step tapes (stage states + gradients) of all K steps '
                                'kept in HBM' % args.n,
-                   'n_cells': nC, 'n_faces': nF, 'dt': dt, 'objective': 'kinetic energy on the stage-1 state'},
+                   'n_cells': nC, 'n_faces': nF, 'n_owner_records': nO, 'dt': dt,
+                   'objective': 'kinetic energy on the stage-1 state'},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': (achieved / (HBM_PEAK / 1e9)) if achieved else None, 'traffic': traffic,
                      'algorithmic_bytes_per_launch': kb, 'avg_launch_ms': avg_s * 1e3},
