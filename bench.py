@@ -93,9 +93,12 @@ def run_precision(case, u0, prec, K, W, dt, torch, profile, recompute=False):
     pa_pass(min(W, K))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    prof = pa_pass(K, prof=profile)
+    pa_pass(K)
     torch.cuda.synchronize()
     t_pa = time.perf_counter() - t0
+    # per-kernel times from HIP events around every launch, in a pass of
+    # their own: the event records lengthen a step by about 4%
+    prof = pa_pass(K, prof=True) if profile else None
     ok = bool(torch.isfinite(u).all()) and bool(torch.isfinite(ua).all())
     # primal only
     torch.cuda.synchronize()
