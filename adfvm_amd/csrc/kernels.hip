@@ -89,17 +89,27 @@ template <typename T> struct DMesh {
     const T* bcval;
 };
 
+// Loads and stores of data a launch touches once, in rows whose cache lines
+// one instruction consumes whole (face geometry, slot tables, a thread's own
+// state rows): non-temporal, so that they are first out of L2 and the cell
+// records that neighbouring threads read again stay in it (grad 6%, face
+// flux 5%, gradient adjoint 8% faster at TGV 128^3 FP32).  Not for records
+// that straddle lines or that lanes share: there the hint re-fetches lines
+// (adjoint gather 55% slower with non-temporal record loads).
+template <typename T> __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T> __device__ __forceinline__ void stnt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 template <typename T> __device__ __forceinline__ void load_geo(const DMesh<T>& m, int f, FaceGeo<T>& g) {
     const T* G = m.geo;
     const long s = m.nF;
-    g.A = G[G_A * s + f];
-    g.n[0] = G[G_NX * s + f]; g.n[1] = G[G_NY * s + f]; g.n[2] = G[G_NZ * s + f];
-    g.w = G[G_W * s + f];
-    g.delta = G[G_DELTA * s + f];
-    g.D[0] = G[G_DX * s + f]; g.D[1] = G[G_DY * s + f]; g.D[2] = G[G_DZ * s + f];
-    g.lw0 = G[G_LW0 * s + f]; g.lw1 = G[G_LW1 * s + f];
-    g.qw0[0] = G[G_Q0X * s + f]; g.qw0[1] = G[G_Q0Y * s + f]; g.qw0[2] = G[G_Q0Z * s + f];
-    g.qw1[0] = G[G_Q1X * s + f]; g.qw1[1] = G[G_Q1Y * s + f]; g.qw1[2] = G[G_Q1Z * s + f];
+    g.A = ldnt(G + G_A * s + f);
+    g.n[0] = ldnt(G + G_NX * s + f); g.n[1] = ldnt(G + G_NY * s + f); g.n[2] = ldnt(G + G_NZ * s + f);
+    g.w = ldnt(G + G_W * s + f);
+    g.delta = ldnt(G + G_DELTA * s + f);
+    g.D[0] = ldnt(G + G_DX * s + f); g.D[1] = ldnt(G + G_DY * s + f); g.D[2] = ldnt(G + G_DZ * s + f);
+    g.lw0 = ldnt(G + G_LW0 * s + f); g.lw1 = ldnt(G + G_LW1 * s + f);
+    g.qw0[0] = ldnt(G + G_Q0X * s + f); g.qw0[1] = ldnt(G + G_Q0Y * s + f); g.qw0[2] = ldnt(G + G_Q0Z * s + f);
+    g.qw1[0] = ldnt(G + G_Q1X * s + f); g.qw1[1] = ldnt(G + G_Q1Y * s + f); g.qw1[2] = ldnt(G + G_Q1Z * s + f);
 }
 
 // primitive values of a value source: internal cell (from the state) or ghost.
@@ -215,12 +225,12 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
     T g[15];
     for (int i = 0; i < 15; i++) g[i] = T(0);
     for (int j = 0; j < 6; j++) {
-        const int o = m.sOther[j * nIC + c];
+        const int o = ldnt(m.sOther + j * nIC + c);
         T wo[5];
         load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
         const T* sg = m.sgeo + (long)(4 * j) * nIC + c;
-        const T SN[3] = {sg[0], sg[nIC], sg[2 * nIC]};
-        const T w = sg[3 * nIC];
+        const T SN[3] = {ldnt(sg), ldnt(sg + nIC), ldnt(sg + 2 * nIC)};
+        const T w = ldnt(sg + 3 * nIC);
         for (int i = 0; i < 5; i++) {
             T phiF = wc[i] * (T(1) - w) + wo[i] * w;
             g[3 * i + 0] += phiF * SN[0];
@@ -296,8 +306,8 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
     if (f >= m.nF) return;
     const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
     const int P = m.owner[f];
-    const int o = m.fOther[f];
-    const int kind = m.fKind[f];
+    const int o = ldnt(m.fOther + f);
+    const int kind = ldnt(m.fKind + f);
     FaceGeo<T> fg;
     load_geo(m, f, fg);
     T wP[5], gP[GREC], wN[5], gN[GREC], F[5], lam;
@@ -337,8 +347,8 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
         T L[5] = {0, 0, 0, 0, 0};
         T lamSum = 0;
         for (int j = 0; j < 6; j++) {
-            const int f = m.sFace[j * nIC + c];
-            const bool own = m.sFlags[j * nIC + c] & 1;
+            const int f = ldnt(m.sFace + j * nIC + c);
+            const bool own = ldnt(m.sFlags + j * nIC + c) & 1;
             T rec[FREC];
             vload<FREC>(fbuf + (long)FREC * f, rec);
             for (int i = 0; i < 5; i++) L[i] += own ? rec[i] : -rec[i];
@@ -352,9 +362,9 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
             for (int i = 0; i < 5; i++) uout[i * nIC + c] = L[i];
         } else {
             for (int i = 0; i < 5; i++) {
-                T acc = ai * ui[i * nIC + c];
-                if (u0) acc = a0 * u0[i * nIC + c] + acc;
-                T Li = S ? L[i] - S[i * nIC + c] : L[i];
+                T acc = ai * ldnt(ui + i * nIC + c);
+                if (u0) acc = a0 * ldnt(u0 + i * nIC + c) + acc;
+                T Li = S ? L[i] - ldnt(S + i * nIC + c) : L[i];
                 uout[i * nIC + c] = acc + (nbeta * Li) * dt;
             }
         }
