@@ -87,6 +87,8 @@ SIGNATURES = {
     'adfvm_kernel_config': (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_double)]),
     'adfvm_set_kernel_block': (C.c_int, [vp, C.c_char_p, C.c_int32]),
+    'adfvm_set_streaming': (C.c_int, [vp, C.c_int32]),
+    'adfvm_streaming': (C.c_int, [vp]),
     'adfvm_primal_step': (C.c_int, [vp, vp, C.c_double, vp, vp, vp, vp, vp]),
     'adfvm_tape_bytes': (C.c_int64, [vp]),
     'adfvm_adjoint_step': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp, vp, vp]),

@@ -369,6 +369,16 @@ class Context(object):
     def set_kernel_block(self, name, block):
         self._chk(self.lib.adfvm_set_kernel_block(self.handle, name.encode(), int(block)))
 
+    def set_streaming(self, mode):
+        """Non-temporal loads of single-use rows: True, False, or None for
+        the automatic choice (on when a stage's working set exceeds the
+        256 MB Infinity Cache).  Results do not depend on it."""
+        self._chk(self.lib.adfvm_set_streaming(self.handle, -1 if mode is None else int(bool(mode))))
+
+    @property
+    def streaming(self):
+        return bool(self.lib.adfvm_streaming(self.handle))
+
     def pack(self, rho, rhoU, rhoE, out=None, stream=None):
         out = out if out is not None else self.state()
         self._chk(self.lib.adfvm_pack_state(self.handle, _dptr(rho), _dptr(rhoU), _dptr(rhoE), _dptr(out),

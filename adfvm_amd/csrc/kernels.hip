@@ -96,20 +96,30 @@ template <typename T> struct DMesh {
 // flux 5%, gradient adjoint 8% faster at TGV 128^3 FP32).  Not for records
 // that straddle lines or that lanes share: there the hint re-fetches lines
 // (adjoint gather 55% slower with non-temporal record loads).
-template <typename T> __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
-template <typename T> __device__ __forceinline__ void stnt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+// ST selects it per context (adfvm_ctx::stream): on meshes whose stage
+// working set fits the 256 MB Infinity Cache the same rows are reused by the
+// next kernels and stages, and plain loads keep them there (the 512 x 256
+// cylinder ran 9% slower streamed).
+template <bool ST, typename T> __device__ __forceinline__ T ld(const T* p) {
+    if constexpr (ST) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool ST, typename T> __device__ __forceinline__ void st(T* p, T v) {
+    if constexpr (ST) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
-template <typename T> __device__ __forceinline__ void load_geo(const DMesh<T>& m, int f, FaceGeo<T>& g) {
+template <bool ST, typename T> __device__ __forceinline__ void load_geo(const DMesh<T>& m, int f, FaceGeo<T>& g) {
     const T* G = m.geo;
     const long s = m.nF;
-    g.A = ldnt(G + G_A * s + f);
-    g.n[0] = ldnt(G + G_NX * s + f); g.n[1] = ldnt(G + G_NY * s + f); g.n[2] = ldnt(G + G_NZ * s + f);
-    g.w = ldnt(G + G_W * s + f);
-    g.delta = ldnt(G + G_DELTA * s + f);
-    g.D[0] = ldnt(G + G_DX * s + f); g.D[1] = ldnt(G + G_DY * s + f); g.D[2] = ldnt(G + G_DZ * s + f);
-    g.lw0 = ldnt(G + G_LW0 * s + f); g.lw1 = ldnt(G + G_LW1 * s + f);
-    g.qw0[0] = ldnt(G + G_Q0X * s + f); g.qw0[1] = ldnt(G + G_Q0Y * s + f); g.qw0[2] = ldnt(G + G_Q0Z * s + f);
-    g.qw1[0] = ldnt(G + G_Q1X * s + f); g.qw1[1] = ldnt(G + G_Q1Y * s + f); g.qw1[2] = ldnt(G + G_Q1Z * s + f);
+    g.A = ld<ST>(G + G_A * s + f);
+    g.n[0] = ld<ST>(G + G_NX * s + f); g.n[1] = ld<ST>(G + G_NY * s + f); g.n[2] = ld<ST>(G + G_NZ * s + f);
+    g.w = ld<ST>(G + G_W * s + f);
+    g.delta = ld<ST>(G + G_DELTA * s + f);
+    g.D[0] = ld<ST>(G + G_DX * s + f); g.D[1] = ld<ST>(G + G_DY * s + f); g.D[2] = ld<ST>(G + G_DZ * s + f);
+    g.lw0 = ld<ST>(G + G_LW0 * s + f); g.lw1 = ld<ST>(G + G_LW1 * s + f);
+    g.qw0[0] = ld<ST>(G + G_Q0X * s + f); g.qw0[1] = ld<ST>(G + G_Q0Y * s + f); g.qw0[2] = ld<ST>(G + G_Q0Z * s + f);
+    g.qw1[0] = ld<ST>(G + G_Q1X * s + f); g.qw1[1] = ld<ST>(G + G_Q1Y * s + f); g.qw1[2] = ld<ST>(G + G_Q1Z * s + f);
 }
 
 // primitive values of a value source: internal cell (from the state) or ghost.
@@ -214,7 +224,7 @@ __global__ void __launch_bounds__(BLOCK) ghost_kernel(DMesh<T> m, Phys<T> ph, co
 }
 
 // Gauss cell gradients of U, T, p (op.gradCell, op.py:45-63).
-template <typename T, int BS>
+template <typename T, int BS, bool ST>
 __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                      const T* __restrict__ ghost, T* __restrict__ grad) {
     int c = xcd_block() * BS + threadIdx.x;
@@ -225,12 +235,12 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
     T g[15];
     for (int i = 0; i < 15; i++) g[i] = T(0);
     for (int j = 0; j < 6; j++) {
-        const int o = ldnt(m.sOther + j * nIC + c);
+        const int o = ld<ST>(m.sOther + j * nIC + c);
         T wo[5];
         load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
         const T* sg = m.sgeo + (long)(4 * j) * nIC + c;
-        const T SN[3] = {ldnt(sg), ldnt(sg + nIC), ldnt(sg + 2 * nIC)};
-        const T w = ldnt(sg + 3 * nIC);
+        const T SN[3] = {ld<ST>(sg), ld<ST>(sg + nIC), ld<ST>(sg + 2 * nIC)};
+        const T w = ld<ST>(sg + 3 * nIC);
         for (int i = 0; i < 5; i++) {
             T phiF = wc[i] * (T(1) - w) + wo[i] * w;
             g[3 * i + 0] += phiF * SN[0];
@@ -298,7 +308,7 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(const double* __restric
 // Internal and coupled faces: RCF.flux (density.py:253-305) between the
 // owner and the neighbour (for a cyclic face, the paired internal cell);
 // characteristic / other boundary faces: the ghost as the right state.
-template <typename T, int BS>
+template <typename T, int BS, bool ST>
 __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                           const T* __restrict__ grad, const T* __restrict__ ghost,
                                                           T* __restrict__ fbuf) {
@@ -306,10 +316,10 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
     if (f >= m.nF) return;
     const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
     const int P = m.owner[f];
-    const int o = ldnt(m.fOther + f);
-    const int kind = ldnt(m.fKind + f);
+    const int o = ld<ST>(m.fOther + f);
+    const int kind = ld<ST>(m.fKind + f);
     FaceGeo<T> fg;
-    load_geo(m, f, fg);
+    load_geo<ST>(m, f, fg);
     T wP[5], gP[GREC], wN[5], gN[GREC], F[5], lam;
     load_prim(ph, u, nIC, ghost, nBF, P, wP);
     load_grad(grad, nIC, P, gP);
@@ -334,7 +344,7 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
 //   lam_c = (1/V_c) sum_slots (lam A)_f      (op.absDiv, op.py:22-29)
 // MODE 0: uout = a0*u0 + ai*ui + (-beta*(L-S))*dt   (timestep.update, timestep.py:35-45)
 // MODE 1: uout = L (the residual)
-template <typename T, int MODE, int BS>
+template <typename T, int MODE, int BS, bool ST>
 __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restrict__ fbuf,
                                                       const T* __restrict__ ui, const T* __restrict__ u0, T a0,
                                                       T ai, T nbeta, T dt, const T* __restrict__ S,
@@ -347,8 +357,8 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
         T L[5] = {0, 0, 0, 0, 0};
         T lamSum = 0;
         for (int j = 0; j < 6; j++) {
-            const int f = ldnt(m.sFace + j * nIC + c);
-            const bool own = ldnt(m.sFlags + j * nIC + c) & 1;
+            const int f = ld<ST>(m.sFace + j * nIC + c);
+            const bool own = ld<ST>(m.sFlags + j * nIC + c) & 1;
             T rec[FREC];
             vload<FREC>(fbuf + (long)FREC * f, rec);
             for (int i = 0; i < 5; i++) L[i] += own ? rec[i] : -rec[i];
@@ -362,9 +372,9 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
             for (int i = 0; i < 5; i++) uout[i * nIC + c] = L[i];
         } else {
             for (int i = 0; i < 5; i++) {
-                T acc = ai * ldnt(ui + i * nIC + c);
-                if (u0) acc = a0 * ldnt(u0 + i * nIC + c) + acc;
-                T Li = S ? L[i] - ldnt(S + i * nIC + c) : L[i];
+                T acc = ai * ld<ST>(ui + i * nIC + c);
+                if (u0) acc = a0 * ld<ST>(u0 + i * nIC + c) + acc;
+                T Li = S ? L[i] - ld<ST>(S + i * nIC + c) : L[i];
                 uout[i * nIC + c] = acc + (nbeta * Li) * dt;
             }
         }
@@ -639,6 +649,9 @@ struct adfvm_ctx {
     // internal faces sorted by owner: per block size, the face ranges of the
     // owner-summing face adjoint blocks (no owner split between two ranges)
     bool ownerSum = false;
+    // the streaming policy (ld / st): non-temporal single-use rows when one
+    // stage's working set exceeds the Infinity Cache (adfvm_set_streaming)
+    bool stream = false;
     int* fvjpBlk[4] = {nullptr, nullptr, nullptr, nullptr};
     int fvjpNBlk[4] = {0, 0, 0, 0};
     void* fbuf = nullptr;      // [6][nF] face fluxes * area
@@ -708,6 +721,24 @@ template <typename F> void with_block(int bs, F&& f) {
         case 512: f(std::integral_constant<int, 512>()); break;
         default: f(std::integral_constant<int, 256>()); break;
     }
+}
+
+// Calls f(std::bool_constant<st>): the streaming policy of the kernels that
+// take one (ld / st).
+template <typename F> void with_stream(bool st, F&& f) {
+    if (st) f(std::bool_constant<true>());
+    else f(std::bool_constant<false>());
+}
+
+// The automatic streaming policy: on when one RK stage's working set
+// exceeds the 256 MB Infinity Cache.  Per face: 17 geometry rows, the face
+// tables (9 B) and a flux record (8 scalars); per cell: state, gradient
+// record, slot geometry, volume, three stage states (61 scalars) and the
+// slot tables (30 B).
+bool stream_auto(const adfvm_ctx* c) {
+    const double s = c->prec;
+    const double bytes = (double)c->nF * (25 * s + 9) + (double)c->nIC * (61 * s + 30);
+    return bytes > 256.0 * (1 << 20);
 }
 
 template <typename T> Phys<T> phys_cast(const Phys<double>& p) {
@@ -842,18 +873,24 @@ template <typename T> void launch_ghosts(adfvm_ctx* c, const T* u, hipStream_t s
 template <typename T> void launch_grad(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
     KTimer kt(c, "grad_kernel", s);
     with_block(c->bs[K_GRAD], [&](auto B) {
-        constexpr int BS = decltype(B)::value;
-        grad_kernel<T, BS><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost),
-                                                               grad);
+        with_stream(c->stream, [&](auto S) {
+            constexpr int BS = decltype(B)::value;
+            constexpr bool ST = decltype(S)::value;
+            grad_kernel<T, BS, ST><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u,
+                                                                       P<T>(c->ghost), grad);
+        });
     });
 }
 
 template <typename T> void launch_face_flux(adfvm_ctx* c, const T* u, const T* grad, hipStream_t s) {
     KTimer kt(c, "face_flux_kernel", s);
     with_block(c->bs[K_FLUX], [&](auto B) {
-        constexpr int BS = decltype(B)::value;
-        face_flux_kernel<T, BS><<<grid_bs(c->nF, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, grad,
-                                                                   P<T>(c->ghost), P<T>(c->fbuf));
+        with_stream(c->stream, [&](auto S) {
+            constexpr int BS = decltype(B)::value;
+            constexpr bool ST = decltype(S)::value;
+            face_flux_kernel<T, BS, ST><<<grid_bs(c->nF, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u,
+                                                                           grad, P<T>(c->ghost), P<T>(c->fbuf));
+        });
     });
 }
 
@@ -864,10 +901,13 @@ void launch_stage(adfvm_ctx* c, const T* ui, const T* u0, const Stage& st, doubl
                   T* dtcCell, double* dtcPartial, hipStream_t s) {
     KTimer kt(c, MODE == 0 ? "stage_kernel" : "residual_kernel", s);
     with_block(c->bs[K_STAGE], [&](auto B) {
-        constexpr int BS = decltype(B)::value;
-        stage_kernel<T, MODE, BS><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.a0,
-                                                                      (T)st.ai, (T)(-st.beta), (T)dt, S, out,
-                                                                      dtcCell, dtcPartial);
+        with_stream(c->stream, [&](auto SP) {
+            constexpr int BS = decltype(B)::value;
+            constexpr bool ST = decltype(SP)::value;
+            stage_kernel<T, MODE, BS, ST><<<grid_bs(c->nIC, BS), BS, 0, s>>>(
+                dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.a0, (T)st.ai, (T)(-st.beta), (T)dt, S, out, dtcCell,
+                dtcPartial);
+        });
     });
 }
 
@@ -1145,6 +1185,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     rc = c->prec == 4 ? create_typed<float>(c, d) : create_typed<double>(c, d);
     if (rc != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->patches, c->hpatches)) != ADFVM_OK) return rc;
+    c->stream = stream_auto(c);
     c->nPartial = grid_bs(std::max<long>(nIC, nF), kBlockChoices[0]) + 1;   // the smallest block size
     HIPCHK(c, hipMalloc((void**)&c->partial, c->nPartial * sizeof(double)));
     HIPCHK(c, hipMalloc((void**)&c->scal, 16 * sizeof(double)));

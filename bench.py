@@ -107,8 +107,10 @@ def run_precision(case, u0, prec, K, W, dt, torch, profile, recompute=False):
         ctx.primal_step(u, dt)
     torch.cuda.synchronize()
     t_p = time.perf_counter() - t0
+    streaming = ctx.streaming
     del ctx
-    return dict(t_pa=t_pa, t_p=t_p, prof=prof, finite=ok, tuned={k: v[0] for k, v in tuned.items()})
+    return dict(t_pa=t_pa, t_p=t_p, prof=prof, finite=ok, tuned={k: v[0] for k, v in tuned.items()},
+                streaming=streaming)
 
 
 def l2_error(case, u0, steps, dt, torch):
@@ -355,6 +357,7 @@ def main():
         'value_fp32_recompute_stages': nC * K / r32r['t_pa'] if r32r else None,
         'kernel_ms_fp32': {k: round(v[0] / K, 4) for k, v in prof.items()},
         'block_sizes_fp32': r32['tuned'],
+        'streaming_fp32': r32['streaming'],
         'finite': r32['finite'],
     }
     if r64 is not None:

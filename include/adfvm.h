@@ -284,6 +284,15 @@ int adfvm_kernel_config(const adfvm_ctx* ctx, int32_t i, char* name, int32_t len
                         double* ms);
 /* Sets one tunable kernel's block size (64, 128, 256 or 512) by name. */
 int adfvm_set_kernel_block(adfvm_ctx* ctx, const char* name, int32_t block);
+/* Streaming policy of the stencil kernels: mode 1 loads the rows a launch
+ * uses once (face and slot geometry, slot tables, a cell's own state)
+ * non-temporal, so that they leave L2 first; 0 loads them plainly, keeping
+ * them cached for the next kernels; -1 (the default) chooses 1 when one RK
+ * stage's working set exceeds 256 MB, the MI355X Infinity Cache.  Results
+ * do not depend on it.  No counterpart in the reference (a cache policy). */
+int adfvm_set_streaming(adfvm_ctx* ctx, int32_t mode);
+/* The policy in use: 1 streaming, 0 not, negative for a null context. */
+int adfvm_streaming(const adfvm_ctx* ctx);
 
 /* Operator-level entry points (the hot path's pieces, used by tests). */
 

@@ -1,6 +1,7 @@
-"""Kernel block sizes (adfvm_autotune / adfvm_set_kernel_block) change
-launch shapes only: the step and its adjoint are bitwise identical under
-every block size, and the autotuner picks one of the offered sizes."""
+"""Kernel block sizes (adfvm_autotune / adfvm_set_kernel_block) and the
+streaming policy (adfvm_set_streaming) change launch shapes and cache hints
+only: the step and its adjoint are bitwise identical under every block size
+and either policy, and the autotuner picks one of the offered sizes."""
 import numpy as np
 import pytest
 
@@ -46,3 +47,26 @@ def test_block_sizes_bitwise(name, prec):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
     with pytest.raises(Exception):
         ctx.set_kernel_block('grad_kernel', 96)
+
+
+@pytest.mark.parametrize('name', ['tgv6', 'mixed_upt', 'mixed_totalpt_lf', 'mixed_drag'])
+@pytest.mark.parametrize('prec', [np.float32, np.float64])
+def test_streaming_policy_bitwise(name, prec):
+    """Non-temporal loads of single-use rows (on or off) leave every result
+    bit for bit the same: primal step, dtc, objective and adjoint step
+    (cyclic, ghost and characteristic faces, the drag objective).  Small meshes default
+    to plain loads (their stage working set fits the Infinity Cache)."""
+    case, u0 = CASES[name]()
+    ctx = case.context(prec)
+    assert ctx.streaming is False
+    u = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
+    dt = 1e-6
+    ref = step_and_adjoint(ctx, u, dt)
+    ctx.set_streaming(True)
+    assert ctx.streaming is True
+    got = step_and_adjoint(ctx, u, dt)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert got[2] == ref[2] and got[3] == ref[3]
+    ctx.set_streaming(None)
+    assert ctx.streaming is False
+    assert ctx.lib.adfvm_set_streaming(ctx.handle, 2) != 0      # modes are -1, 0, 1
