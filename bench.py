@@ -59,7 +59,7 @@ def kernel_bytes(name, s, nC, nF, nG, nO=None):
 
 
 def step_bytes_model(s, nC, nF, nG):
-    """Primal and primal+adjoint bytes per cell-update, the survey's model:
+    """Primal and primal+adjoint bytes per cell-update, the step-level byte model:
     B_P = s[nC(5+5+10/3+15+15+1) + 40 nG + 17 nF] + 8 nF per stage,
     B_A = s[nC(5+15+5+5+20/3+15+15+1) + 40 nG + 17 nF] + 8 nF per stage."""
     BP = s * (nC * (5 + 5 + 10. / 3 + 15 + 15 + 1) + 40 * nG + 17 * nF) + 8 * nF
