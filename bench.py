@@ -132,6 +132,26 @@ def l2_error(case, u0, steps, dt, torch):
     return err
 
 
+def triad_bandwidth(torch, n=1 << 28, reps=10):
+    """Device-memory bandwidth a plain stream reaches: c = a + 0.5 b on three
+    n-element FP32 arrays (1 GiB each by default), reads 2n and writes n
+    scalars per launch, timed with HIP events on torch's stream (GB/s)."""
+    a = torch.ones(n, dtype=torch.float32, device='cuda')
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
+    for _ in range(3):
+        torch.add(a, b, alpha=0.5, out=c)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.add(a, b, alpha=0.5, out=c)
+    e1.record()
+    torch.cuda.synchronize()
+    s = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b, c
+    return 3 * n * 4 / s / 1e9
+
+
 def cpu_baseline(n=48, steps=20, dt=2e-4):
     """The oracle (torch float64 on the host, autograd reverse) on a bounded
     sample: TGV n^3, `steps` steps forward + reverse."""
@@ -328,6 +348,7 @@ def main():
     tf = os.path.join(ROOT, 'profiles', 'traffic.json')
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(dom)
+    triad = triad_bandwidth(torch)
     bpc_p32, bpc_pa32 = step_bytes_model(4, nC, nF, nG)
     bpc_p64, bpc_pa64 = step_bytes_model(8, nC, nF, nG)
     out = {
@@ -350,8 +371,10 @@ def main():
                    'objective': 'kinetic energy on the stage-1 state'},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': (achieved / (HBM_PEAK / 1e9)) if achieved else None, 'traffic': traffic,
-                     'algorithmic_bytes_per_launch': kb, 'avg_launch_ms': avg_s * 1e3},
+                     'algorithmic_bytes_per_launch': kb, 'avg_launch_ms': avg_s * 1e3,
+                     'triad_GBs': triad, 'frac_of_triad': (achieved / triad) if achieved else None},
         'hbm_frac_step_model': value * bpc_pa32 / HBM_PEAK,
+        'triad_frac_step_model': value * bpc_pa32 / (triad * 1e9),
         'bytes_per_cell_update_model': {'fp32_primal': bpc_p32, 'fp32_pa': bpc_pa32, 'fp64_pa': bpc_pa64},
         'primal_cell_updates_per_s_fp32': nC * K / r32['t_p'],
         'value_fp32_recompute_stages': nC * K / r32r['t_pa'] if r32r else None,
