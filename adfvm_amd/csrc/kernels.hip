@@ -648,6 +648,9 @@ struct adfvm_ctx {
     int nPlanes = 6;           // adjoint record planes: one per cell slot, then cyclic extras
     // internal faces sorted by owner: per block size, the face ranges of the
     // owner-summing face adjoint blocks (no owner split between two ranges)
+    // boundary faces with a ghost by face kind (0 characteristic, 1 boundary)
+    int* kindFaces[2] = {nullptr, nullptr};
+    int nKindFaces[2] = {0, 0};
     bool ownerSum = false;
     // the streaming policy (ld / st): non-temporal single-use rows when one
     // stage's working set exceeds the Infinity Cache (adfvm_set_streaming)
@@ -1136,6 +1139,14 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     }
     c->nPlanes = 6 + maxExtra;
     int rc;
+    for (int k = 0; k < 2; k++) {
+        const int kind = k == 0 ? FACE_CHARACTERISTIC : FACE_BOUNDARY;
+        std::vector<int> list;
+        for (long b = 0; b < nBF; b++)
+            if (fKind[nIF + b] == kind) list.push_back((int)b);
+        c->nKindFaces[k] = (int)list.size();
+        if ((rc = upload(c, (void**)&c->kindFaces[k], list)) != ADFVM_OK) return rc;
+    }
     // The face adjoint sums a cell's owner sides in LDS when one owner's
     // internal faces are consecutive (owners non-decreasing over the internal
     // faces) and are the cell's first slots, in face order: the sum is then
@@ -1199,7 +1210,8 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
                     c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->ngeo, c->visc,
-                    c->fvjpBlk[0], c->fvjpBlk[1], c->fvjpBlk[2], c->fvjpBlk[3]};
+                    c->fvjpBlk[0], c->fvjpBlk[1], c->fvjpBlk[2], c->fvjpBlk[3], c->kindFaces[0],
+                    c->kindFaces[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
