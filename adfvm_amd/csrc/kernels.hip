@@ -174,22 +174,16 @@ __device__ __forceinline__ void load_grad(const T* __restrict__ grad, long nIC, 
 
 // ------------------------------------------------------------- kernels ---
 
-// Ghost primitive values of non-cyclic boundary faces (Solver.boundary +
-// characteristicBoundary, solver.py:106-114, density.py:333-337, BCs.py).
+// Ghost primitive values of non-cyclic boundary face b from its owner's
+// primitive values w (Solver.boundary + characteristicBoundary,
+// solver.py:106-114, density.py:333-337, BCs.py): every rule reads only the
+// owner, the face normal and the patch's values, so the kernels that need a
+// ghost evaluate it where they use it.
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) ghost_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
-                                                      T* __restrict__ ghost) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m.nBF) return;
+__device__ __forceinline__ void ghost_value(const DMesh<T>& m, const Phys<T>& ph, int b, const T* w, T* out) {
     const DPatch P = m.patches[m.bPatch[b]];
-    if (P.bcU == ADFVM_BC_CYCLIC && P.cbc == ADFVM_CBC_NONE) return;   // resolved by the slot tables
     const int f = m.nIF + b;
     const int k = f - P.start;
-    const long nIC = m.nIC, nBF = m.nBF;
-    const int c = m.owner[f];
-    T q[5], w[5], out[5];
-    for (int i = 0; i < 5; i++) q[i] = u[i * nIC + c];
-    primitive(ph, q, w);
     const T* V = m.bcval;
     if (P.cbc == ADFVM_CBC_UPT) {                    // BCs.py:152-161
         for (int i = 0; i < 3; i++) out[i] = V[P.valU + 3 * k + i];
@@ -220,11 +214,28 @@ __global__ void __launch_bounds__(BLOCK) ghost_kernel(DMesh<T> m, Phys<T> ph, co
         out[3] = (P.bcT == ADFVM_BC_FIXED_VALUE || P.bcT == ADFVM_BC_CALCULATED) ? V[P.valT + k] : w[3];
         out[4] = (P.bcp == ADFVM_BC_FIXED_VALUE || P.bcp == ADFVM_BC_CALCULATED) ? V[P.valp + k] : w[4];
     }
+}
+
+// The ghost array (ghost primitives [5][nBF]), for the consumers that read
+// it as an array: the ghost-value entry point and the adjoint viscosity.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) ghost_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
+                                                      T* __restrict__ ghost) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nBF) return;
+    const DPatch P = m.patches[m.bPatch[b]];
+    if (P.bcU == ADFVM_BC_CYCLIC && P.cbc == ADFVM_CBC_NONE) return;   // resolved by the slot tables
+    const long nIC = m.nIC, nBF = m.nBF;
+    const int c = m.owner[m.nIF + b];
+    T q[5], w[5], out[5];
+    for (int i = 0; i < 5; i++) q[i] = u[i * nIC + c];
+    primitive(ph, q, w);
+    ghost_value(m, ph, b, w, out);
     for (int i = 0; i < 5; i++) ghost[i * nBF + b] = out[i];
 }
 
 // Gauss cell gradients of U, T, p (op.gradCell, op.py:45-63).
-template <typename T, int BS, bool ST>
+template <typename T, int BS, bool ST, bool GH>
 __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                      const T* __restrict__ ghost, T* __restrict__ grad) {
     int c = xcd_block() * BS + threadIdx.x;
@@ -237,7 +248,16 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
     for (int j = 0; j < 6; j++) {
         const int o = ld<ST>(m.sOther + j * nIC + c);
         T wo[5];
-        load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
+        if constexpr (GH) {
+            // a ghost slot's value from this cell (the face's owner)
+            T wi[5], wg[5];
+            load_prim(ph, u, nIC, u, nIC, o < 0 ? c : o, wi);
+            for (int i = 0; i < 5; i++) wg[i] = wc[i];
+            if (o < 0) ghost_value(m, ph, -1 - o, wc, wg);
+            for (int i = 0; i < 5; i++) wo[i] = o < 0 ? wg[i] : wi[i];
+        } else {
+            load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
+        }
         const T* sg = m.sgeo + (long)(4 * j) * nIC + c;
         const T SN[3] = {ld<ST>(sg), ld<ST>(sg + nIC), ld<ST>(sg + 2 * nIC)};
         const T w = ld<ST>(sg + 3 * nIC);
@@ -308,7 +328,7 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(const double* __restric
 // Internal and coupled faces: RCF.flux (density.py:253-305) between the
 // owner and the neighbour (for a cyclic face, the paired internal cell);
 // characteristic / other boundary faces: the ghost as the right state.
-template <typename T, int BS, bool ST>
+template <typename T, int BS, bool ST, bool GH>
 __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
                                                           const T* __restrict__ grad, const T* __restrict__ ghost,
                                                           T* __restrict__ fbuf) {
@@ -323,7 +343,16 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
     T wP[5], gP[GREC], wN[5], gN[GREC], F[5], lam;
     load_prim(ph, u, nIC, ghost, nBF, P, wP);
     load_grad(grad, nIC, P, gP);
-    load_prim(ph, u, nIC, ghost, nBF, o, wN);
+    if constexpr (GH) {
+        // a boundary face's ghost from its owner
+        T wi[5], wg[5];
+        load_prim(ph, u, nIC, u, nIC, o < 0 ? P : o, wi);
+        for (int i = 0; i < 5; i++) wg[i] = wP[i];
+        if (o < 0) ghost_value(m, ph, -1 - o, wP, wg);
+        for (int i = 0; i < 5; i++) wN[i] = o < 0 ? wg[i] : wi[i];
+    } else {
+        load_prim(ph, u, nIC, ghost, nBF, o, wN);
+    }
     if (kind == FACE_INTERIOR) {
         load_grad(grad, nIC, o, gN);
         face_flux(ph, FACE_INTERIOR, wP, gP, wN, gN, fg, F, &lam);
@@ -441,17 +470,18 @@ __global__ void __launch_bounds__(BLOCK) finalize_seg_kernel(const double* __res
 
 // Drag on one patch (templates/cylinder_test.py:9-19): sum (p0 nx - mu(T0)(U0x-Uix)/delta) A.
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) drag_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u,
-                                                     const T* __restrict__ ghost, int start, int n, double* partial) {
+__global__ void __launch_bounds__(BLOCK) drag_kernel(DMesh<T> m, Phys<T> ph, const T* __restrict__ u, int start,
+                                                     int n, double* partial) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
     if (k < n) {
         const int f = start + k;
         const int b = f - m.nIF;
-        const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
-        T wi[5];
-        load_prim(ph, u, nIC, ghost, nBF, m.owner[f], wi);
-        T U0 = ghost[b], p0 = ghost[4 * nBF + b], T0 = ghost[3 * nBF + b];
+        const long nIC = m.nIC, nF = m.nF;
+        T wi[5], wg[5];
+        load_prim(ph, u, nIC, u, nIC, m.owner[f], wi);
+        ghost_value(m, ph, b, wi, wg);
+        T U0 = wg[0], p0 = wg[4], T0 = wg[3];
         T nx = m.geo[G_NX * nF + f];
         T mungUx = viscosity(ph, T0) * (U0 - wi[0]) / m.geo[G_DELTA * nF + f];
         v = (double)((p0 * nx - mungUx) * m.geo[G_A * nF + f]);
@@ -733,6 +763,14 @@ template <typename F> void with_stream(bool st, F&& f) {
     else f(std::bool_constant<false>());
 }
 
+// Calls f(std::bool_constant<gh>): whether the mesh has faces with a ghost
+// (the kernels that evaluate ghosts in place are built without that code
+// for meshes that have none).
+template <typename F> void with_ghosts(bool gh, F&& f) {
+    if (gh) f(std::bool_constant<true>());
+    else f(std::bool_constant<false>());
+}
+
 // The automatic streaming policy: on when one RK stage's working set
 // exceeds the 256 MB Infinity Cache.  Per face: 17 geometry rows, the face
 // tables (9 B) and a flux record (8 scalars); per cell: state, gradient
@@ -877,10 +915,12 @@ template <typename T> void launch_grad(adfvm_ctx* c, const T* u, T* grad, hipStr
     KTimer kt(c, "grad_kernel", s);
     with_block(c->bs[K_GRAD], [&](auto B) {
         with_stream(c->stream, [&](auto S) {
-            constexpr int BS = decltype(B)::value;
-            constexpr bool ST = decltype(S)::value;
-            grad_kernel<T, BS, ST><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u,
-                                                                       P<T>(c->ghost), grad);
+            with_ghosts(c->hasGhostKernel, [&](auto G) {
+                constexpr int BS = decltype(B)::value;
+                constexpr bool ST = decltype(S)::value, GH = decltype(G)::value;
+                grad_kernel<T, BS, ST, GH><<<grid_bs(c->nIC, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd),
+                                                                               u, P<T>(c->ghost), grad);
+            });
         });
     });
 }
@@ -889,10 +929,12 @@ template <typename T> void launch_face_flux(adfvm_ctx* c, const T* u, const T* g
     KTimer kt(c, "face_flux_kernel", s);
     with_block(c->bs[K_FLUX], [&](auto B) {
         with_stream(c->stream, [&](auto S) {
-            constexpr int BS = decltype(B)::value;
-            constexpr bool ST = decltype(S)::value;
-            face_flux_kernel<T, BS, ST><<<grid_bs(c->nF, BS), BS, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u,
-                                                                           grad, P<T>(c->ghost), P<T>(c->fbuf));
+            with_ghosts(c->hasGhostKernel, [&](auto G) {
+                constexpr int BS = decltype(B)::value;
+                constexpr bool ST = decltype(S)::value, GH = decltype(G)::value;
+                face_flux_kernel<T, BS, ST, GH><<<grid_bs(c->nF, BS), BS, 0, s>>>(
+                    dmesh<T>(c), phys_cast<T>(c->phd), u, grad, P<T>(c->ghost), P<T>(c->fbuf));
+            });
         });
     });
 }
@@ -938,7 +980,7 @@ template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* ou
     } else if (c->objective == ADFVM_OBJ_DRAG) {
         const DPatch& Pp = c->hpatches[c->objectivePatch];
         int g = grid_for(Pp.n);
-        drag_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, P<T>(c->ghost), Pp.start, Pp.n, c->partial);
+        drag_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, Pp.start, Pp.n, c->partial);
         finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
     } else {
         fill_kernel<<<1, 1, 0, s>>>(out, c->nSeg, 0.0);
@@ -980,7 +1022,6 @@ int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, do
         // entry); a single-stage scheme reads u's neighbours, so it goes
         // through u1 and is copied back
         T* out = (i == ns - 1 && ns > 1) ? u : bufs[i];
-        launch_ghosts<T>(c, ui, s);
         if (i == objStage && obj_out) launch_objective<T>(c, ui, obj_out, s);
         launch_grad<T>(c, ui, grads[i], s);
         const bool wantDtc = (i == objStage) && dtc_out;
@@ -1296,21 +1337,18 @@ extern "C" int adfvm_primal_step(adfvm_ctx* c, void* u, double dt, const void* S
 
 namespace {
 template <typename T> int gradients_t(adfvm_ctx* c, const T* u, T* grad, hipStream_t s) {
-    launch_ghosts<T>(c, u, s);
     launch_grad<T>(c, u, P<T>(c->grad0), s);
     grad_rows_kernel<T><<<grid_for(c->nIC), BLOCK, 0, s>>>(c->nIC, P<T>(c->grad0), grad);
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
 }
 template <typename T> int residual_t(adfvm_ctx* c, const T* u, T* R, T* dtcCell, hipStream_t s) {
-    launch_ghosts<T>(c, u, s);
     launch_grad<T>(c, u, P<T>(c->grad0), s);
     launch_flux_and_stage<T, 1>(c, u, P<T>(c->grad0), nullptr, Stage{0, 0, 0}, 0.0, nullptr, R, dtcCell, nullptr, s);
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
 }
 template <typename T> int objective_t(adfvm_ctx* c, const T* u, double* out, hipStream_t s) {
-    launch_ghosts<T>(c, u, s);
     launch_objective<T>(c, u, out, s);
     HIPCHK(c, hipGetLastError());
     return ADFVM_OK;
