@@ -5,11 +5,12 @@
 //    scalars in the context precision (float or double);
 //  * every face->cell accumulation is a gather: each cell walks its six
 //    cellFaces slots in order (owned faces in face order, then neighbour
-//    faces, cmesh.cpp:9-19) and computes the face quantities it needs.  No
-//    atomics: results are bit-identical from run to run;
-//  * cyclic ghosts are resolved at set-up to the paired internal cell, so a
-//    periodic mesh needs no ghost storage; non-cyclic boundary faces get a
-//    small ghost array of primitive values refreshed once per stage;
+//    faces, cmesh.cpp:9-19) and reads what the face kernel stored for them.
+//    No atomics: results are bit-identical from run to run;
+//  * cyclic ghosts are resolved at set-up to the paired internal cell; the
+//    ghost of a non-cyclic boundary face is evaluated where it is used, from
+//    its owner cell and the patch's values (a ghost array is filled only for
+//    the ghost-value entry point and the adjoint viscosity);
 //  * reductions (objective, dtc) write one partial per block and a single
 //    block folds the partials in index order, in FP64.
 #include <hip/hip_runtime.h>
