@@ -70,3 +70,22 @@ def test_streaming_policy_bitwise(name, prec):
     ctx.set_streaming(None)
     assert ctx.streaming is False
     assert ctx.lib.adfvm_set_streaming(ctx.handle, 2) != 0      # modes are -1, 0, 1
+
+
+def test_streaming_policy_automatic_choice():
+    """The automatic policy streams when one stage's working set (face
+    geometry, face tables and flux records; cell state, gradient record,
+    slot geometry, volume, stage states and slot tables) exceeds 256 MB:
+    TGV 64^3 is about 159 MB in FP32 (plain loads) and 303 MB in FP64
+    (streamed).  Either policy gives the same step bit for bit there too."""
+    case, params = cf.tgv(64, seed=0)
+    u0 = cf.tgv_state(case, params)
+    for prec, want in ((np.float32, False), (np.float64, True)):
+        ctx = case.context(prec)
+        assert ctx.streaming is want
+        u = torch.as_tensor(u0.astype(prec), device='cuda').contiguous()
+        ref = step_and_adjoint(ctx, u, 1e-5)
+        ctx.set_streaming(not want)
+        got = step_and_adjoint(ctx, u, 1e-5)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        del ctx
