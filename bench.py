@@ -58,6 +58,22 @@ def kernel_bytes(name, s, nC, nF, nG, nO=None):
     return None
 
 
+def kernel_rooflines(prof, s, nC, nF, nG, nO, triad):
+    """Per kernel: average launch (ms, HIP events), algorithmic bytes per
+    launch, their rate (GB/s) and its fraction of the 8 TB/s peak and of the
+    measured triad bandwidth (GB/s)."""
+    out = {}
+    for k, (ms_tot, launches) in prof.items():
+        b = kernel_bytes(k, s, nC, nF, nG, nO)
+        if not b or not launches:
+            continue
+        ms = ms_tot / launches
+        gbs = b / (ms * 1e-3) / 1e9
+        out[k] = {'avg_launch_ms': round(ms, 4), 'algorithmic_bytes': int(b), 'GBs': round(gbs, 1),
+                  'frac_peak': round(gbs / (HBM_PEAK / 1e9), 3), 'frac_triad': round(gbs / triad, 3)}
+    return out
+
+
 def step_bytes_model(s, nC, nF, nG):
     """Primal and primal+adjoint bytes per cell-update, the step-level byte model:
     B_P = s[nC(5+5+10/3+15+15+1) + 40 nG + 17 nF] + 8 nF per stage,
@@ -379,6 +395,7 @@ def main():
         'primal_cell_updates_per_s_fp32': nC * K / r32['t_p'],
         'value_fp32_recompute_stages': nC * K / r32r['t_pa'] if r32r else None,
         'kernel_ms_fp32': {k: round(v[0] / K, 4) for k, v in prof.items()},
+        'kernel_roofline_fp32': kernel_rooflines(prof, 4, nC, nF, nG, nO, triad),
         'block_sizes_fp32': r32['tuned'],
         'streaming_fp32': r32['streaming'],
         'finite': r32['finite'],

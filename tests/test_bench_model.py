@@ -47,3 +47,14 @@ def test_more_than_one_gpu_is_refused():
                         '--warmup', '0'], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert 'one GPU' in (r.stdout + r.stderr)
+
+
+def test_kernel_rooflines():
+    nC, nF, nG = SIZES['tgv128']
+    prof = {'face_vjp_kernel': (4.3, 10), 'grad_kernel': (1.0, 10), 'unknown_kernel': (1.0, 10)}
+    r = bench.kernel_rooflines(prof, 4, nC, nF, nG, nC - 1, 6000.0)
+    assert set(r) == {'face_vjp_kernel', 'grad_kernel'}
+    b = bench.kernel_bytes('face_vjp_kernel', 4, nC, nF, nG, nC - 1)
+    assert r['face_vjp_kernel']['algorithmic_bytes'] == int(b)
+    assert abs(r['face_vjp_kernel']['GBs'] - b / 0.43e-3 / 1e9) < 0.1
+    assert abs(r['face_vjp_kernel']['frac_peak'] - b / 0.43e-3 / 8e12) < 1e-3
