@@ -136,3 +136,24 @@ def test_reference_layout_lists():
     assert len(s) == 8 + 3 * 6
     assert m.areas.shape == (m.nFaces, 1) and m.quadraticWeights.shape == (m.nFaces, 2, 3)
     assert m.faces.shape[1] == 5 and (m.faces[:, 0] == 4).all()
+
+
+def test_patch_face_cell_ranges():
+    """getPatchFaceCellRange (used by adFVM/field.py:358,386): a patch's
+    faces [startFace, startFace + nFaces) and their ghost cells, numbered
+    nInternalCells + (face - nInternalFaces) (adFVM/mesh.py:244,773)."""
+    m = Mesh.from_data(meshgen.box((3, 2, 2)))
+    total = 0
+    for pid, b in m.boundary.items():
+        s, e, cs, ce, n = m.getPatchFaceCellRange(pid)
+        assert (s, e, n) == (b['startFace'], b['startFace'] + b['nFaces'], b['nFaces'])
+        assert cs == m.nInternalCells + (s - m.nInternalFaces) and ce - cs == n
+        total += n
+    assert total == m.nFaces - m.nInternalFaces
+
+
+def test_time_function_decorator(capsys):
+    from adfvm_amd import config
+    f = config.timeFunction('Time for adding')(lambda a, b=1: a + b)
+    assert f(2, b=3) == 5
+    assert capsys.readouterr().out.startswith('Time for adding:')
