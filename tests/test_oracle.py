@@ -552,3 +552,47 @@ def test_viscosity_heat_step_against_direct_solve():
     assert np.allclose(vo.jacobi(nd, cd, nb, ones, 7), 3.5, rtol=1e-14)
     Mu = vo.scaled_viscosity(c, 'uniform', None, 2.5)
     assert np.allclose(Mu, 2.5, rtol=1e-15)
+
+
+# ---------------------------------------------------------- TGV integrals --
+def _tgv_ke_rates(n):
+    """Kinetic energy of the TGV initial field and its rate of change under
+    the oracle's residual, with the case's viscosity and with none."""
+    from tests import casefactory as cf
+    case, params = cf.tgv(n, seed=0)
+    u = cf.tgv_state(case, params)
+    V = case.mesh.volumes[:, 0]
+    rho, U = u[0], u[1:4] / u[0]
+    ke = float((0.5 * rho * (U ** 2).sum(0)) @ V)
+    rates = []
+    for mu in (case.mu0, 0.0):
+        oc = cf.CaseDef(case.md, {}, mu).oracle()
+        t = lambda a: torch.as_tensor(a, dtype=torch.float64)
+        d0, d1, _, _ = oc.equation(t(u[0:1].T), t(u[1:4].T), t(u[4:5].T))
+        drho, drU = -d0.T.numpy()[0], -d1.T.numpy()      # the residual is -du/dt
+        rates.append(float(((U * drU).sum(0) - 0.5 * (U ** 2).sum(0) * drho) @ V))
+    return ke, rates[0] - rates[1], rates[1], params, case.mu0
+
+
+def test_tgv_kinetic_energy_and_dissipation_closed_forms():
+    """Taylor-Green u = U0 sin x cos y cos z, v = -U0 cos x sin y cos z on
+    [0, 2 pi]^3: KE = rho0 pi^3 U0^2 (the O(M^2) density fluctuation
+    integrates to zero against |U|^2, and midpoint sums of these products
+    are exact); the viscous part of dKE/dt is -mu int |grad u|^2 =
+    -6 pi^3 mu U0^2 (the transposed and divergence terms integrate to zero
+    for this periodic divergence-free field), reached at second order (the
+    difference of the rates with and without viscosity removes the Roe
+    scheme's own dissipation); and that numerical dissipation, the inviscid
+    rate, falls at third order with the mesh spacing, as a second-order
+    upwind-biased reconstruction's does."""
+    ke24, visc24, inv24, p, mu = _tgv_ke_rates(24)
+    ke32, visc32, inv32, _, _ = _tgv_ke_rates(32)
+    U0, rho0 = p['U0'], p['rho0']
+    for ke in (ke24, ke32):
+        assert abs(ke / (rho0 * np.pi ** 3 * U0 ** 2) - 1) < 1e-12
+    exact = -6 * np.pi ** 3 * mu * U0 ** 2
+    e24, e32 = abs(visc24 / exact - 1), abs(visc32 / exact - 1)
+    assert e24 < 5e-3 and e32 < 3e-3
+    assert 1.5 < e24 / e32 < 2.3                     # (32/24)^2 = 1.78
+    assert inv24 < 0 and inv32 < 0
+    assert abs((inv32 / inv24) / (24 / 32) ** 3 - 1) < 0.1
