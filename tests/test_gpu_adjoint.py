@@ -276,6 +276,47 @@ def test_bc_value_gradient_vs_central_fd_multistep():
         assert rel <= 1e-6, (field, patch, key, rel)
 
 
+@pytest.mark.parametrize('name', ['mixed_upt', 'mixed_totalpt_lf', 'mixed_calc', 'mixed_drag'])
+def test_adjoint_taylor_remainder_second_order(name):
+    """Taylor remainder of one FP64 step's functional
+    J(u) = <w, step(u)> + obja * objective(u):
+    r(eps) = |J(u + eps v) - J(u) - eps <grad J, v>| with grad J from the
+    adjoint step must fall as eps^2 (ratio 4 per halving); a gradient that
+    is wrong in any direction leaves an O(eps) remainder (ratio 2).  The
+    perturbation scale keeps r far above FP64 round-off and below the
+    distance to the entropy fix's and limiter's branch points on these
+    generic states (every ghost rule, Roe and LF, the drag objective).  The
+    Taylor-Green adjoint is left to the central-difference tests: its face
+    jumps |UnL - UnR| are so small that an eps on one side of every kink of
+    the entropy fix leaves remainders at FP64 round-off."""
+    case, u = adjoint_case(name)
+    ctx = case.context(np.float64)
+    rng = np.random.default_rng(21)
+    scale = np.abs(u).mean(1, keepdims=True)
+    v = rng.normal(size=u.shape) * scale
+    w = rng.normal(size=u.shape) * scale * 1e-3
+    dt = 1e-6 if name.startswith('mixed') else 2e-5
+    obja = 1.0 if case.objective else 0.0
+
+    def J(x):
+        xd = torch.as_tensor(x, device='cuda').contiguous()
+        obj = torch.zeros(1, dtype=torch.float64, device='cuda')
+        ctx.primal_step(xd, dt, obj_out=obj)
+        return float((xd.cpu().numpy() * w).sum()) + obja * obj.item()
+
+    ud = torch.as_tensor(u, device='cuda').contiguous()
+    ua = torch.as_tensor(w, device='cuda').contiguous()
+    ctx.adjoint_step(ud, dt, ua, obja=obja)
+    dJ = float((ua.cpu().numpy() * v).sum())
+    J0 = J(u)
+    eps = [1e-4 * 2.0 ** -k for k in range(4)]
+    r = [abs(J(u + e * v) - J0 - e * dJ) for e in eps]
+    ratios = [r[k] / r[k + 1] for k in range(3)]
+    print('%s Taylor remainders %s, ratios %s' % (name, ' '.join('%.3e' % x for x in r),
+                                                  ' '.join('%.2f' % x for x in ratios)))
+    assert all(3.5 < q < 4.5 for q in ratios)
+
+
 def _degenerate_cases():
     """Edge cases of the index tables: periodic boxes of 1 cell (all six
     faces cyclic, the cell its own partner: six extra record planes) and 2
