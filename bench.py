@@ -224,6 +224,56 @@ def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
             'objective_range': [float(J.min()), float(J.max())]}
 
 
+def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
+    """Taylor-Green n^3 adjoint: n_steps fixed-dt steps (CFL 0.5) primal
+    plus the adjoint of the kinetic energy summed over the steps (the
+    reference's objective hook on the stage-1 state), uniform checkpoints
+    every `interval` steps, each checkpoint segment one HIP graph; FP32
+    kernels with FP64 objective accumulation, and the same run in FP64 for
+    the error and speed ratio."""
+    from adfvm_amd.graphs import GraphSweep
+    from tests import casefactory as cf
+    t0 = time.time()
+    case, params = cf.tgv(n, seed=0)
+    u0h = cf.tgv_state(case, params)
+    nC = case.mesh.nInternalCells
+    c0 = np.sqrt(cf.GAMMA * cf.R * params['T0'])
+    dt = 0.5 * (2 * np.pi / n) / (params['U0'] + c0)
+    print('[bench] TGV %d^3 adjoint, %d cells, built in %.1fs' % (n, nC, time.time() - t0), file=sys.stderr,
+          flush=True)
+    res, tuned = {}, None
+    for prec in (np.float32, np.float64):
+        ctx = case.context(prec)
+        u0 = torch.as_tensor(u0h.astype(prec), device='cuda').contiguous()
+        if prec == np.float32:
+            tuned = ctx.autotune(u0, reps=3)
+        sweep = GraphSweep(ctx, dt, interval, obja=1.0)
+        sweep.run(u0, interval)                    # capture + one segment
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J, ua = sweep.run(u0, n_steps)
+        t = time.perf_counter() - t1
+        res[prec] = (float(J[0]), ua.double().cpu().numpy(), t)
+        print('[bench] TGV %d^3 adjoint %s: %.3f s, J %.10e' % (n, np.dtype(prec).name, t, J[0]), file=sys.stderr,
+              flush=True)
+        del sweep, ctx
+    j32, g32, t32 = res[np.float32]
+    j64, g64, t64 = res[np.float64]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {'metric': 'cell-updates/s primal+adjoint, FP32 vs FP64, % HBM roofline, L2 err vs FP64',
+            'value': nC * n_steps / t32, 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': n_steps, 'warmup': interval,
+            'ms_per_step': t32 / n_steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': 'f32', 'data': 'synthetic (Taylor-Green initial field)',
+            'config': {'workload': 'TGV %d^3 periodic hex, random permutation + RCM, %d steps primal + adjoint of '
+                                   'the summed kinetic energy, checkpoints every %d steps, HIP-graph segments'
+                                   % (n, n_steps, interval), 'n_cells': nC, 'dt': dt},
+            'block_sizes': {k: v[0] for k, v in tuned.items()},
+            'value_fp64': nC * n_steps / t64, 'fp32_over_fp64': t64 / t32,
+            'objective_fp32': j32, 'objective_fp64': j64, 'objective_rel_err_fp32_vs_fp64': abs(j32 - j64) / abs(j64),
+            'grad_rel_l2_fp32_vs_fp64': {'rho': rel(g32[0:1], g64[0:1]), 'rhoU': rel(g32[1:4], g64[1:4]),
+                                         'rhoE': rel(g32[4:5], g64[4:5])}}
+
+
 def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, ratio=1.02):
     """2-D cylinder wake: O-grid of n_theta x n_r x 1 hex cells, radial
     spacing growing by `ratio` from r = 0.5 D to 20 D, Mach 0.2, Re = 100,
@@ -292,6 +342,9 @@ def main():
     ap.add_argument('--ensemble', action='store_true',
                     help='instead: the batched ensemble (64 Taylor-Green instances of 32^3..96^3, 200 steps '
                          'primal+adjoint, checkpoints every 20, HIP-graph segments)')
+    ap.add_argument('--adjoint64', action='store_true',
+                    help='instead: the Taylor-Green 64^3 adjoint (400 steps primal + adjoint of the summed kinetic '
+                         'energy, checkpoints every 20, HIP-graph segments), FP32 and FP64')
     ap.add_argument('--cylinder', action='store_true',
                     help='instead: the 2D cylinder wake (512 x 256 O-grid, 5000 steps primal + adjoint of the mean '
                          'drag, checkpoints every 100, HIP-graph segments), FP32 and FP64')
@@ -311,6 +364,9 @@ def main():
         return
     if args.cylinder:
         print(json.dumps(cylinder_bench(torch)))
+        return
+    if args.adjoint64:
+        print(json.dumps(adjoint64_bench(torch)))
         return
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
