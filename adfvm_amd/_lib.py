@@ -92,6 +92,8 @@ SIGNATURES = {
     'adfvm_primal_step': (C.c_int, [vp, vp, C.c_double, vp, vp, vp, vp, vp]),
     'adfvm_tape_bytes': (C.c_int64, [vp]),
     'adfvm_adjoint_step': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp, vp, vp]),
+    'adfvm_adjoint_step_cotangent': (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_double, vp, vp, vp, vp, vp]),
+    'adfvm_primitive_ghosts_vjp': (C.c_int, [vp, vp, vp, vp, vp, vp]),
     'adfvm_set_bc_values': (C.c_int, [vp, C.c_int32, C.c_int32, vp, C.c_int64, vp]),
     'adfvm_adjoint_viscosity': (C.c_int, [vp, vp, vp, C.c_double, C.c_int32, C.c_double, C.c_int32, vp, vp]),
     'adfvm_gradients': (C.c_int, [vp, vp, vp, vp]),
@@ -108,6 +110,7 @@ SIGNATURES = {
     'adfvm_profile_entry': (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]),
     'adfvm_profile_reset': (C.c_int, [vp]),
+    'adfvm_source_hash': (C.c_char_p, []),
     'adfvm_pack_state': (C.c_int, [vp, vp, vp, vp, vp, vp]),
     'adfvm_unpack_state': (C.c_int, [vp, vp, vp, vp, vp, vp]),
 }
@@ -120,8 +123,10 @@ class LibraryError(RuntimeError):
 
 
 def load():
-    """Load libadfvm.so (once).  Raises if it is absent: there is no
-    non-native path."""
+    """Load libadfvm.so (once).  Raises if it is absent, or if it was built
+    from sources other than the csrc/ and include/ files next to it (the
+    SHA-256 the build embeds): there is no non-native path, and no stale
+    one."""
     global _lib
     if _lib is not None:
         return _lib
@@ -129,6 +134,14 @@ def load():
         raise LibraryError('libadfvm.so not built: run `python -c "import __graft_entry__ as g; g.build()"` '
                            'or `python -m adfvm_amd.build` (expected at %s)' % LIB_PATH)
     lib = C.CDLL(LIB_PATH)
+    from .build import source_hash
+    if not hasattr(lib, 'adfvm_source_hash'):
+        raise LibraryError('%s predates the source check: rebuild it (python -m adfvm_amd.build)' % LIB_PATH)
+    lib.adfvm_source_hash.restype = C.c_char_p
+    built, now = lib.adfvm_source_hash().decode(), source_hash()
+    if built != now:
+        raise LibraryError('%s was built from other sources (hash %s..., the sources give %s...): rebuild it '
+                           '(python -m adfvm_amd.build)' % (LIB_PATH, built[:12], now[:12]))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res

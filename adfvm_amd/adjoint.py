@@ -40,7 +40,7 @@ class DeviceStepper(object):
     step n is reversed."""
 
     def __init__(self, ctx, dts, ua, source=None, source_grad=None, obja=1.0, tapes=True, after_reverse=None,
-                 bc_grad=None):
+                 bc_grad=None, solver=None):
         self.ctx = ctx
         self.dts = dts
         self.ua = ua
@@ -51,6 +51,10 @@ class DeviceStepper(object):
         self.tapes = tapes
         self.after_reverse = after_reverse
         self.forwards = 0
+        # a solver whose objective is a case-file function: its cotangent
+        # enters each reversed step (the library has no kernel for it)
+        from .objectives import PythonObjective
+        self.solver = solver if solver is not None and isinstance(solver.objective, PythonObjective) else None
 
     def clone(self, u):
         return u.clone()
@@ -62,14 +66,20 @@ class DeviceStepper(object):
         return tape
 
     def reverse(self, u_n, n, tape):
-        self.ctx.adjoint_step(u_n, self.dts[n], self.ua, obja=self.obja(n), source=self.source,
-                              source_grad=self.source_grad, bc_grad=self.bc_grad, tape=tape)
+        og, w = None, self.obja(n)
+        if self.solver is not None:
+            from .objectives import stage_cotangent
+            og = stage_cotangent(self.ctx, self.solver, u_n, self.dts[n], self.source, tape, w, self.bc_grad)
+            if og is not None:
+                w = 1.0
+        self.ctx.adjoint_step(u_n, self.dts[n], self.ua, obja=w, source=self.source,
+                              source_grad=self.source_grad, bc_grad=self.bc_grad, tape=tape, obj_grad=og)
         if self.after_reverse is not None:
             self.after_reverse(n, u_n)
 
 
 def sweep(ctx, u0, dts, schedule='store_all', interval=None, snapshots=None, obja=1.0, ua=None, source=None,
-          source_grad=None, tapes=True, after_reverse=None, checkpoints=None, bc_grad=None):
+          source_grad=None, tapes=True, after_reverse=None, checkpoints=None, bc_grad=None, solver=None):
     """Reverse nSteps = len(dts) steps from the device state u0 (unchanged).
 
     Returns ua = d/du0 of sum_n obja(n) J_n + <ua_in, u_N> (ua_in = the given
@@ -78,11 +88,13 @@ def sweep(ctx, u0, dts, schedule='store_all', interval=None, snapshots=None, obj
     (5, nBoundaryFaces), Context.bc_state) accumulate the source and
     boundary-value gradients of the same functional.  schedule: 'store_all', 'uniform'
     (interval steps per segment; `checkpoints` may hold segment starts
-    n -> state) or 'revolve' (`snapshots` extra states)."""
+    n -> state) or 'revolve' (`snapshots` extra states).  solver: an RCF
+    whose objective is a case-file function objective(fields, solver) (the
+    context then has no objective of its own); J_n is then that function's."""
     n = len(dts)
     if ua is None:
         ua = ctx.state(0.)
-    st = DeviceStepper(ctx, dts, ua, source, source_grad, obja, tapes, after_reverse, bc_grad)
+    st = DeviceStepper(ctx, dts, ua, source, source_grad, obja, tapes, after_reverse, bc_grad, solver)
     if schedule == 'store_all':
         checkpoint.store_all(st, u0, 0, n)
     elif schedule == 'uniform':
@@ -133,6 +145,15 @@ class Adjoint(object):
         self.forceReadFields = False
         self.homogeneousAdjoint = False
         self.ua = None
+
+    def compileSolver(self, parameters=('source',)):
+        """The positional adjoint Functions of apps/adjoint.py:94-141:
+        self.map (primal_grad) and self.viscousMap (primal_grad_viscous,
+        when a viscosityType is set), for the parameter parameters[0]."""
+        param = list(parameters)[0] if parameters else 'source'
+        self.map = AdjointMap(self, param)
+        self.viscousMap = AdjointMap(self, param, viscous=True) if self.viscosityType else None
+        return self
 
     # ------------------------------------------------------------- fields --
     def createFields(self):
@@ -238,6 +259,8 @@ class Adjoint(object):
         energy = torch.zeros(1, dtype=torch.float64, device='cuda')
         sensTimeSeries, energyTimeSeries = [], []
         obja = 0. if self.homogeneousAdjoint else 1.
+        from .objectives import PythonObjective, stage_cotangent
+        pyobj = isinstance(primal.objective, PythonObjective)
 
         totalCheckpoints = nSteps // writeInterval
         nCheckpoints = min(firstCheckpoint + runCheckpoints, totalCheckpoints)
@@ -265,8 +288,12 @@ class Adjoint(object):
             energyTimeSeries.append(energy.item() ** 0.5)
             for r in range(1, writeInterval + 1):              # reverse count, apps/adjoint.py:244-249
                 k = writeInterval - r
-                ctx.adjoint_step(states[k], dts[k], ua, obja=obja, source=primal.source, source_grad=sgrad,
-                                 bc_grad=bgrad, tape=tps[k])
+                og, w = None, obja
+                if pyobj:
+                    og = stage_cotangent(ctx, primal, states[k], dts[k], primal.source, tps[k], obja, bgrad)
+                    w = 1.0 if og is not None else obja
+                ctx.adjoint_step(states[k], dts[k], ua, obja=w, source=primal.source, source_grad=sgrad,
+                                 bc_grad=bgrad, tape=tps[k], obj_grad=og)
                 if self.viscosityType and r % viscousInterval == 0:       # apps/adjoint.py:250,288-289
                     ctx.adjoint_viscosity(states[k], ua, dts[k], self.viscosityType, self.scaling,
                                           iterations=self.viscousIterations)
@@ -302,3 +329,85 @@ class Adjoint(object):
         if self.finished:
             self.removeStatusFile()
         return result
+
+
+class AdjointMap(object):
+    """The primal_grad Function's call contract (apps/adjoint.py:94-126,
+    called at :268-291): positional primal inputs (as PrimalMap takes them)
+    + [rhoa (nIC,1), rhoUa (nIC,3), rhoEa (nIC,1), dtca (1,1), obja (1,1)]
+    + [scaling (1,1)]; returns [rhoa', rhoUa', rhoEa'] + the parameter
+    gradients: the 3 source arrays' for 'source' (apps/adjoint.py:104-106),
+    the value array's (nFaces, components) for ('BCs', field, patch, key)
+    (:111-118).  Adjoint fields are times volume, as the reference keeps
+    them (apps/adjoint.py:185-189).  The parameter gradients accumulate over
+    calls (the reference's static outputs): return_static returns them,
+    zero_static zeroes them after (both default to True); with
+    return_static False their slots hold None.  dtca must be 0 (the
+    reference always seeds it so, :268); a non-zero seed raises.  With
+    viscous=True the adjoint-viscosity smoothing of the Adjoint's
+    viscosityType follows the step (the viscous map, :127-140) with the
+    passed scaling."""
+
+    OPTIONS = ('return_static', 'zero_static', 'return_reusable', 'replace_reusable')
+
+    def __init__(self, adjoint, parameter='source', viscous=False):
+        from .density import PrimalMap
+        primal = adjoint.primal
+        self.adjoint, self.primal, self.ctx = adjoint, primal, primal.ctx
+        self.pmap = PrimalMap(primal)
+        self.parameter = parameter
+        self.viscous = viscous
+        if not (parameter == 'source' or (isinstance(parameter, tuple) and len(parameter) == 4
+                                          and parameter[0] == 'BCs')):
+            raise NotImplementedError('parameter %r (source and BC values are supported)' % (parameter,))
+        if parameter != 'source':
+            _, phi, patchID, key = parameter
+            self.rows, self.faces = self.ctx.bc_param(phi, patchID, key)
+        self.sgrad = self.ctx.state(0.)
+        self.bgrad = self.ctx.bc_state()
+
+    def __call__(self, *args, **options):
+        for k in options:
+            if k not in self.OPTIONS:
+                raise TypeError('unknown option %r' % k)
+        ret_static = options.get('return_static', True)
+        zero_static = options.get('zero_static', True)
+        pmap, ctx = self.pmap, self.ctx
+        n_primal = 4 + pmap.inputs.n
+        if len(args) != n_primal + 6:
+            raise ValueError('expected %d positional arguments (%d primal inputs, 5 adjoint inputs, scaling), got %d'
+                             % (n_primal + 6, n_primal, len(args)))
+        rho, rhoU, rhoE, dt = args[:4]
+        source = pmap.inputs.consume(args[4:n_primal])
+        rhoa, rhoUa, rhoEa, dtca, obja, scaling = args[n_primal:]
+        if float(np.asarray(dtca).reshape(-1)[0]) != 0.0:
+            raise NotImplementedError('a non-zero dtc seed (the reference seeds 0, apps/adjoint.py:268)')
+        obja = float(np.asarray(obja).reshape(-1)[0])
+        dt = float(np.asarray(dt).reshape(-1)[0])
+        u = pmap.state(rho, rhoU, rhoE)
+        ua = pmap.state(rhoa, rhoUa, rhoEa)
+        primal = self.primal
+        og, w = None, obja
+        from .objectives import PythonObjective, stage_cotangent
+        if isinstance(primal.objective, PythonObjective):
+            og = stage_cotangent(ctx, primal, u, dt, source, None, obja, self.bgrad)
+            w = 1.0 if og is not None else obja
+        ctx.adjoint_step(u, dt, ua, obja=w, source=source, source_grad=self.sgrad, bc_grad=self.bgrad, obj_grad=og)
+        if self.viscous and self.adjoint.viscosityType:
+            ctx.adjoint_viscosity(u, ua, dt, self.adjoint.viscosityType,
+                                  float(np.asarray(scaling).reshape(-1)[0]),
+                                  iterations=self.adjoint.viscousIterations)
+        r, rU, rE = ctx.unpack(ua)
+        out = [r.cpu().numpy(), rU.cpu().numpy(), rE.cpu().numpy()]
+        if ret_static:
+            if self.parameter == 'source':
+                g = self.sgrad.double().cpu().numpy()
+                out += [g[0:1].T.copy(), g[1:4].T.copy(), g[4:5].T.copy()]
+            else:
+                out.append(self.bgrad[self.rows, self.faces].double().cpu().numpy().T.copy())
+        else:
+            out += [None] * (3 if self.parameter == 'source' else 1)
+        if zero_static:
+            self.sgrad.zero_()
+            self.bgrad.zero_()
+        return out

@@ -4,6 +4,7 @@ library next to this file.
 
     python -m adfvm_amd.build
 """
+import hashlib
 import os
 import subprocess
 import sys
@@ -30,6 +31,21 @@ HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=' + ARCH, '-ffp-contr
 CXX_FLAGS = ['-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', '-I', INCLUDE]
 
 
+def source_hash():
+    """SHA-256 over the library's sources (every file of csrc/ and the C
+    header), embedded in the build and checked by the loader, so that a
+    library built from other sources is never loaded."""
+    h = hashlib.sha256()
+    files = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC))] + [os.path.join(INCLUDE, 'adfvm.h')]
+    for path in files:
+        if os.path.isfile(path):
+            h.update(os.path.basename(path).encode() + b'\0')
+            with open(path, 'rb') as f:
+                h.update(f.read())
+            h.update(b'\0')
+    return h.hexdigest()
+
+
 def run(cmd):
     print(' '.join(cmd), flush=True)
     subprocess.check_call(cmd)
@@ -41,10 +57,11 @@ def build(verbose=True):
     mesh_obj = os.path.join(BUILD_DIR, 'mesh_build.o')
     run(['g++'] + CXX_FLAGS + ['-c', os.path.join(CSRC, 'mesh_build.cpp'), '-o', mesh_obj])
     objs.append(mesh_obj)
+    hash_def = '-DADFVM_SOURCE_HASH="%s"' % source_hash()
     for src in sorted(os.listdir(CSRC)):
         if src.endswith('.hip'):
             obj = os.path.join(BUILD_DIR, src[:-4] + '.o')
-            run([HIPCC] + HIP_FLAGS + ['-c', os.path.join(CSRC, src), '-o', obj])
+            run([HIPCC] + HIP_FLAGS + [hash_def, '-c', os.path.join(CSRC, src), '-o', obj])
             objs.append(obj)
     tmp = LIB + '.tmp'
     run([HIPCC, '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs)

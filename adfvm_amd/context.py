@@ -135,6 +135,7 @@ class Context(object):
         obj_patch = physics.get('objectivePatch')
         ph.objectivePatch = names.index(obj_patch) if obj_patch is not None else -1
         ph.limiter = int(physics.get('limiter', 0))
+        self.integrator = ph.integrator
         handle = C.c_void_p()
         rc = lib.adfvm_ctx_create(C.byref(desc), C.byref(ph), C.byref(handle))
         self.handle = handle
@@ -158,9 +159,21 @@ class Context(object):
     def _chk(self, rc):
         _lib.check(rc, self.handle)
 
-    def _check(self, t, shape):
-        assert t.is_cuda and t.is_contiguous() and t.dtype == self.tdtype, (t.device, t.dtype)
-        assert tuple(t.shape) == tuple(shape), (tuple(t.shape), shape)
+    def _check(self, t, shape, name='array', dtype=None):
+        """Raises ValueError unless t is a contiguous HIP-device tensor of
+        the given shape and dtype (the context precision by default): the
+        kernels index it with the strides that shape implies."""
+        dtype = self.tdtype if dtype is None else dtype
+        if not hasattr(t, 'dtype') or not hasattr(t, 'shape'):
+            raise ValueError('%s: expected a torch tensor, got %r' % (name, type(t)))
+        if t.dtype != dtype:
+            raise ValueError('%s: dtype %s, the context computes in %s' % (name, t.dtype, dtype))
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError('%s: shape %s, expected %s' % (name, tuple(t.shape), tuple(shape)))
+        if not t.is_contiguous():
+            raise ValueError('%s: not contiguous' % name)
+        if not t.is_cuda:
+            raise ValueError('%s: on %s, expected a HIP device tensor' % (name, t.device))
 
     def state(self, fill=None):
         t = self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
@@ -185,34 +198,71 @@ class Context(object):
         self._chk(self.lib.adfvm_set_segments(self.handle, len(a) - 1, _lib.ptr(a, C.c_int64)))
         self.nSeg = len(a) - 1
 
+    def _check_out64(self, t, n, name):
+        if t is None:
+            return
+        if t.dtype != self.torch.float64 or not t.is_contiguous() or t.numel() < n or not t.is_cuda:
+            raise ValueError('%s: needs a contiguous float64 HIP-device tensor of at least %d entries' % (name, n))
+
+    def _check_tape(self, tape):
+        if tape is None:
+            return
+        if not tape.is_contiguous() or not tape.is_cuda or tape.dtype != self.tdtype or \
+                tape.numel() * tape.element_size() < self.tape_bytes():
+            raise ValueError('tape: needs a contiguous %s HIP-device tensor of %d bytes' % (self.tdtype, self.tape_bytes()))
+
     def primal_step(self, u, dt, source=None, dtc_out=None, obj_out=None, tape=None, stream=None):
-        self._check(u, (5, self.nIC))
-        if obj_out is not None:
-            assert obj_out.numel() >= getattr(self, 'nSeg', 1), 'obj_out needs one double per segment'
+        self._check(u, (5, self.nIC), 'u')
+        self._check_out64(obj_out, getattr(self, 'nSeg', 1), 'obj_out')
+        self._check_out64(dtc_out, 1, 'dtc_out')
         if source is not None:
-            self._check(source, (5, self.nIC))
-        if tape is not None:
-            assert tape.is_contiguous() and tape.numel() * tape.element_size() >= self.tape_bytes()
+            self._check(source, (5, self.nIC), 'source')
+        self._check_tape(tape)
         self._chk(self.lib.adfvm_primal_step(self.handle, _dptr(u), float(dt), _dptr(source), _dptr(dtc_out),
                                              _dptr(obj_out), _dptr(tape), _stream(stream)))
 
     def adjoint_step(self, u_n, dt, ua, obja=1.0, source=None, source_grad=None, bc_grad=None, tape=None,
-                     stream=None):
+                     stream=None, obj_grad=None):
         """bc_grad (device (5, nBoundaryFaces), from bc_state()): the
         gradient with respect to the boundary-condition values is added to
-        it; bc_param() locates one ('BCs', field, patch, key) parameter in it."""
-        self._check(u_n, (5, self.nIC))
-        self._check(ua, (5, self.nIC))
-        for a in (source, source_grad):
+        it; bc_param() locates one ('BCs', field, patch, key) parameter in it.
+        obj_grad (device (5, nIC)): a caller objective's dJ/du at the
+        objective stage (stage_state()), added with weight obja."""
+        if obj_grad is not None:
+            self._check(obj_grad, (5, self.nIC), 'obj_grad')
+        self._check(u_n, (5, self.nIC), 'u_n')
+        self._check(ua, (5, self.nIC), 'ua')
+        for a, nm in ((source, 'source'), (source_grad, 'source_grad')):
             if a is not None:
-                self._check(a, (5, self.nIC))
+                self._check(a, (5, self.nIC), nm)
         if bc_grad is not None:
-            self._check(bc_grad, (5, self.nBF))
-        if tape is not None:
-            assert tape.is_contiguous() and tape.numel() * tape.element_size() >= self.tape_bytes()
+            self._check(bc_grad, (5, self.nBF), 'bc_grad')
+        self._check_tape(tape)
+        if obj_grad is not None:
+            self._chk(self.lib.adfvm_adjoint_step_cotangent(self.handle, _dptr(u_n), float(dt), _dptr(source),
+                                                            _dptr(ua), float(obja), _dptr(obj_grad),
+                                                            _dptr(source_grad), _dptr(bc_grad), _dptr(tape),
+                                                            _stream(stream)))
+            return
         self._chk(self.lib.adfvm_adjoint_step(self.handle, _dptr(u_n), float(dt), _dptr(source), _dptr(ua),
                                               float(obja), _dptr(source_grad), _dptr(bc_grad), _dptr(tape),
                                               _stream(stream)))
+
+    @property
+    def n_stages(self):
+        return 1 if self.integrator == _lib.RK_EULER else 3
+
+    def stage_state(self, u_n, tape):
+        """The state the objective is evaluated on in the step from u_n
+        (density.py:412-413): the first intermediate stage, which the step's
+        tape holds first (a view into it), or u_n itself for a one-stage
+        integrator."""
+        if self.n_stages == 1:
+            return u_n
+        if tape is None:
+            raise ValueError('the objective stage of a multi-stage step is read from its tape')
+        self._check_tape(tape)
+        return tape.reshape(-1)[:5 * self.nIC].view(5, self.nIC)
 
     def adjoint_viscosity(self, u_n, ua, dt, vtype, scaling, iterations=1000, m2norm_out=None, stream=None):
         """Adjoint-viscosity smoothing of ua in place after the adjoint step
@@ -220,10 +270,10 @@ class Context(object):
         'uniform', 'abarbanel' or 'turkel'; scaling the adjoint's scaling
         (adjParams[0]); `iterations` Jacobi sweeps (the reference: 1000).
         m2norm_out (device (nIC,)): the scaled viscosity per cell."""
-        self._check(u_n, (5, self.nIC))
-        self._check(ua, (5, self.nIC))
+        self._check(u_n, (5, self.nIC), 'u_n')
+        self._check(ua, (5, self.nIC), 'ua')
         if m2norm_out is not None:
-            self._check(m2norm_out, (self.nIC,))
+            self._check(m2norm_out, (self.nIC,), 'm2norm_out')
         codes = {'uniform': _lib.VISC_UNIFORM, 'abarbanel': _lib.VISC_ABARBANEL, 'turkel': _lib.VISC_TURKEL}
         if vtype not in codes:
             raise ValueError('adjoint viscosity type %r (uniform, abarbanel, turkel)' % (vtype,))
@@ -282,55 +332,106 @@ class Context(object):
                                                _stream(stream)))
 
     def gradients(self, u, out=None, stream=None):
-        self._check(u, (5, self.nIC))
+        self._check(u, (5, self.nIC), 'u')
         out = out if out is not None else self.torch.empty((15, self.nIC), dtype=self.tdtype, device='cuda')
+        self._check(out, (15, self.nIC), 'out')
         self._chk(self.lib.adfvm_gradients(self.handle, _dptr(u), _dptr(out), _stream(stream)))
         return out
 
     def residual(self, u, out=None, dtc_cell=None, stream=None):
-        self._check(u, (5, self.nIC))
+        self._check(u, (5, self.nIC), 'u')
         out = out if out is not None else self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
+        self._check(out, (5, self.nIC), 'out')
+        if dtc_cell is not None:
+            self._check(dtc_cell, (self.nIC,), 'dtc_cell')
         self._chk(self.lib.adfvm_residual(self.handle, _dptr(u), _dptr(out), _dptr(dtc_cell), _stream(stream)))
         return out
 
     def residual_vjp(self, u, rbar, out=None, stream=None):
+        self._check(u, (5, self.nIC), 'u')
+        self._check(rbar, (5, self.nIC), 'rbar')
         out = out if out is not None else self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
+        self._check(out, (5, self.nIC), 'out')
         self._chk(self.lib.adfvm_residual_vjp(self.handle, _dptr(u), _dptr(rbar), _dptr(out), _stream(stream)))
         return out
 
     def objective(self, u, out=None, stream=None):
+        self._check(u, (5, self.nIC), 'u')
         n = getattr(self, 'nSeg', 1)
         out = out if out is not None else self.torch.zeros(n, dtype=self.torch.float64, device='cuda')
-        assert out.numel() >= n
+        self._check_out64(out, n, 'out')
         self._chk(self.lib.adfvm_objective(self.handle, _dptr(u), _dptr(out), _stream(stream)))
         return out
 
     def primitive_ghosts(self, u, out=None, stream=None):
+        self._check(u, (5, self.nIC), 'u')
         nC = self.mesh.nCells
         out = out if out is not None else self.torch.empty((5, nC), dtype=self.tdtype, device='cuda')
+        self._check(out, (5, nC), 'out')
         self._chk(self.lib.adfvm_primitive_ghosts(self.handle, _dptr(u), _dptr(out), _stream(stream)))
         return out
 
+    def primitive_ghosts_vjp(self, u, prim_bar, out=None, bc_grad=None, stream=None):
+        """(d prim/d u)^T prim_bar for prim = primitive_ghosts(u): prim_bar
+        (5, nCells) -> (5, nIC); the boundary-value part is added to bc_grad
+        (5, nBoundaryFaces) when given."""
+        self._check(u, (5, self.nIC), 'u')
+        self._check(prim_bar, (5, self.mesh.nCells), 'prim_bar')
+        out = out if out is not None else self.torch.empty((5, self.nIC), dtype=self.tdtype, device='cuda')
+        self._check(out, (5, self.nIC), 'out')
+        if bc_grad is not None:
+            self._check(bc_grad, (5, self.nBF), 'bc_grad')
+        self._chk(self.lib.adfvm_primitive_ghosts_vjp(self.handle, _dptr(u), _dptr(prim_bar), _dptr(out),
+                                                      _dptr(bc_grad), _stream(stream)))
+        return out
+
     def grad_cell(self, phi, stream=None):
+        """Gauss gradient of phi (k, nCells: ghost values included) ->
+        (3k, nInternalCells), rows d(phi_i)/dx_j at 3i+j."""
+        if getattr(phi, 'ndim', 0) != 2:
+            raise ValueError('phi: expected (k, nCells)')
         k = phi.shape[0]
+        self._check(phi, (k, self.mesh.nCells), 'phi')
         out = self.torch.empty((3 * k, self.nIC), dtype=self.tdtype, device='cuda')
         self._chk(self.lib.adfvm_grad_cell(self.handle, _dptr(phi), k, _dptr(out), _stream(stream)))
         return out
 
     def interp_second_order(self, phi, grad, swap, stream=None):
+        """interp.secondOrder on every face: phi (k, nCells) and grad
+        (3k, nCells) -- gradients on every cell, ghosts included (grad_cell
+        returns internal rows only; the caller supplies the ghost rows) ->
+        (k, nFaces)."""
+        if getattr(phi, 'ndim', 0) != 2:
+            raise ValueError('phi: expected (k, nCells)')
         k = phi.shape[0]
+        self._check(phi, (k, self.mesh.nCells), 'phi')
+        self._check(grad, (3 * k, self.mesh.nCells), 'grad')
+        if swap not in (0, 1):
+            raise ValueError('swap must be 0 or 1')
         out = self.torch.empty((k, self.mesh.nFaces), dtype=self.tdtype, device='cuda')
         self._chk(self.lib.adfvm_interp_second_order(self.handle, _dptr(phi), _dptr(grad), k, swap, _dptr(out),
                                                      _stream(stream)))
         return out
 
     def dot(self, a, b, out=None, stream=None):
+        """sum_j a[j] b[j] in FP64 (a, b: contiguous device tensors of the
+        context precision with the same number of elements)."""
+        if not hasattr(a, 'numel') or not hasattr(b, 'numel'):
+            raise ValueError('a, b: expected torch tensors')
+        self._check(a, tuple(a.shape), 'a')
+        self._check(b, tuple(b.shape), 'b')
+        if b.numel() != a.numel():
+            raise ValueError('a, b: %d and %d elements' % (a.numel(), b.numel()))
         out = out if out is not None else self.torch.zeros(1, dtype=self.torch.float64, device='cuda')
+        self._check_out64(out, 1, 'out')
         self._chk(self.lib.adfvm_dot(self.handle, _dptr(a), _dptr(b), a.numel(), _dptr(out), _stream(stream)))
         return out
 
     def adjoint_energy(self, ua, u, out=None, stream=None):
+        self._check(ua, (5, self.nIC), 'ua')
+        self._check(u, (5, self.nIC), 'u')
         out = out if out is not None else self.torch.zeros(1, dtype=self.torch.float64, device='cuda')
+        self._check_out64(out, 1, 'out')
         self._chk(self.lib.adfvm_adjoint_energy(self.handle, _dptr(ua), _dptr(u), _dptr(out), _stream(stream)))
         return out
 
@@ -380,12 +481,17 @@ class Context(object):
         return bool(self.lib.adfvm_streaming(self.handle))
 
     def pack(self, rho, rhoU, rhoE, out=None, stream=None):
+        self._check(rho, (self.nIC, 1), 'rho')
+        self._check(rhoU, (self.nIC, 3), 'rhoU')
+        self._check(rhoE, (self.nIC, 1), 'rhoE')
         out = out if out is not None else self.state()
+        self._check(out, (5, self.nIC), 'out')
         self._chk(self.lib.adfvm_pack_state(self.handle, _dptr(rho), _dptr(rhoU), _dptr(rhoE), _dptr(out),
                                             _stream(stream)))
         return out
 
     def unpack(self, u, stream=None):
+        self._check(u, (5, self.nIC), 'u')
         t = self.torch
         rho = t.empty((self.nIC, 1), dtype=self.tdtype, device='cuda')
         rhoU = t.empty((self.nIC, 3), dtype=self.tdtype, device='cuda')

@@ -374,11 +374,14 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
 //   lam_c = (1/V_c) sum_slots (lam A)_f      (op.absDiv, op.py:22-29)
 // MODE 0: uout = a0*u0 + ai*ui + (-beta*(L-S))*dt   (timestep.update, timestep.py:35-45)
 // MODE 1: uout = L (the residual)
+// The last SSP-RK3 stage passes the step's state as both u0 and uout (each
+// thread reads u0[c] before writing uout[c] at the same index), so neither
+// is declared __restrict__.
 template <typename T, int MODE, int BS, bool ST>
 __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restrict__ fbuf,
-                                                      const T* __restrict__ ui, const T* __restrict__ u0, T a0,
+                                                      const T* __restrict__ ui, const T* u0, T a0,
                                                       T ai, T nbeta, T dt, const T* __restrict__ S,
-                                                      T* __restrict__ uout, T* __restrict__ dtcCell,
+                                                      T* uout, T* __restrict__ dtcCell,
                                                       double* __restrict__ dtcPartial) {
     int c = xcd_block() * BS + threadIdx.x;
     const long nIC = m.nIC, nF = m.nF;
@@ -984,7 +987,7 @@ template <typename T> void launch_objective(adfvm_ctx* c, const T* u, double* ou
         drag_kernel<T><<<g, BLOCK, 0, s>>>(dmesh<T>(c), phys_cast<T>(c->phd), u, Pp.start, Pp.n, c->partial);
         finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
     } else {
-        fill_kernel<<<1, 1, 0, s>>>(out, c->nSeg, 0.0);
+        fill_kernel<<<grid_for(c->nSeg), BLOCK, 0, s>>>(out, c->nSeg, 0.0);   // one zero per segment
     }
 }
 
@@ -1262,6 +1265,11 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
 }
 
 extern "C" const char* adfvm_last_error(const adfvm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+#ifndef ADFVM_SOURCE_HASH
+#define ADFVM_SOURCE_HASH "unknown"
+#endif
+extern "C" const char* adfvm_source_hash(void) { return ADFVM_SOURCE_HASH; }
 extern "C" int adfvm_ctx_precision(const adfvm_ctx* c) { return c ? c->prec : 0; }
 
 namespace {

@@ -197,25 +197,143 @@ class RCF(Solver):
         return self
 
 
+def bc_inputs(solver, specs=None):
+    """The boundary-value arrays of the reference's Function inputs, in its
+    order (getBoundaryTensor, solver.py:84-85, field.py:141-146): fields U,
+    T, p; patches in sorted order (mesh.py:100-101); per patch the inputs
+    its condition creates, in creation order (BCs.py:70-184: fixedValue
+    'value'; CBC_UPT U0, T0, p0; CBC_TOTAL_PT Tt, pt[, direction];
+    calculated and the copy rules none).  Returns [(field, patch, key,
+    components)].  The gradient fields' default conditions (density.py:
+    108-110) create none."""
+    from . import BCs
+    specs = specs if specs is not None else BCs.validate(solver.mesh, solver.boundaries)
+    comps = {'value': None, 'U0': 3, 'T0': 1, 'p0': 1, 'Tt': 1, 'pt': 1, 'direction': 3}
+    order = {'fixedValue': ('value',), 'CBC_UPT': ('U0', 'T0', 'p0'), 'CBC_TOTAL_PT': ('Tt', 'pt', 'direction')}
+    out = []
+    for field, k in (('U', 3), ('T', 1), ('p', 1)):
+        for patch in solver.mesh.sortedPatches:
+            spec = specs[patch][field]
+            for key in order.get(spec['type'], ()):
+                if key in spec:
+                    out.append((field, patch, key, comps[key] or k))
+    return out
+
+
+def _live(specs, field, patch, key):
+    """Whether the kernels read this boundary value: a fixedValue's value,
+    or a characteristic condition's values on p (density.py:333-337 applies
+    the p field's characteristic condition; the U and T copies are unused)."""
+    t = specs[patch][field]['type']
+    return t == 'fixedValue' or (field == 'p' and t in ('CBC_UPT', 'CBC_TOTAL_PT'))
+
+
+class PositionalInputs(object):
+    """The reference Function's positional inputs after the state (and dt):
+    mesh.getTensor() + mesh.getScalar() + source terms + boundary-value
+    arrays (density.py:97-105, solver.py:312-317).  The mesh arguments must
+    be the context's mesh (a mesh perturbation, solver.py:254-261, is not
+    supported and raises); the source and boundary values are used: the
+    source is uploaded for the call, boundary values that differ from the
+    context's are written to the device (Context.set_bc_value) and kept."""
+
+    def __init__(self, solver):
+        from . import BCs
+        self.solver = solver
+        self.specs = BCs.validate(solver.mesh, solver.boundaries)
+        self.bcs = bc_inputs(solver, self.specs)
+        m = solver.mesh
+        self.mesh_args = m.getTensor() + m.getScalar()
+        self.n_mesh = len(self.mesh_args)
+        self.n = self.n_mesh + 3 + len(self.bcs)
+        from .BCs import parse_value
+        self.values = {}
+        for f, p, key, k in self.bcs:
+            try:
+                self.values[(f, p, key)] = parse_value(self.specs[p][f][key], m.boundary[p]['nFaces'], k)
+            except ValueError:
+                self.values[(f, p, key)] = None
+        self._source = None
+
+    def consume(self, rest):
+        """Checks and applies rest (everything after the state and dt);
+        returns the device source tensor (or None) for the call."""
+        s = self.solver
+        if len(rest) != self.n:
+            raise ValueError('expected %d positional arguments after the state and dt (%d mesh, 3 source, %d '
+                             'boundary-value arrays), got %d' % (self.n, self.n_mesh, len(self.bcs), len(rest)))
+        for i, (a, b) in enumerate(zip(rest[:self.n_mesh], self.mesh_args)):
+            if a is b:
+                continue
+            if not np.array_equal(np.asarray(a), np.asarray(b)):
+                raise NotImplementedError('mesh argument %d differs from the context\'s mesh: mesh perturbations '
+                                          'are not supported' % i)
+        src = rest[self.n_mesh:self.n_mesh + 3]
+        source = None
+        if any(np.any(np.asarray(x) != 0) for x in src):
+            nIC = s.mesh.nInternalCells
+            host = np.concatenate([np.asarray(src[0], np.float64).reshape(nIC, 1).T,
+                                   np.asarray(src[1], np.float64).reshape(nIC, 3).T,
+                                   np.asarray(src[2], np.float64).reshape(nIC, 1).T])
+            if self._source is None or not np.array_equal(self._source[0], host):
+                torch = s._torch()
+                self._source = (host, torch.as_tensor(host, dtype=s.ctx.tdtype).to('cuda').contiguous())
+            source = self._source[1]
+        for (field, patch, key, k), a in zip(self.bcs, rest[self.n_mesh + 3:]):
+            n = s.mesh.boundary[patch]['nFaces']
+            v = np.asarray(a, np.float64)
+            if v.size not in (k, n * k):
+                raise ValueError('boundary value (%s, %s, %s): %d numbers for %d faces x %d' % (field, patch, key,
+                                                                                            v.size, n, k))
+            v = np.broadcast_to(v.reshape(-1, k), (n, k))
+            cur = self.values[(field, patch, key)]
+            if cur is not None and np.array_equal(cur, v):
+                continue
+            self.values[(field, patch, key)] = np.array(v)
+            if _live(self.specs, field, patch, key):
+                s.ctx.set_bc_value(field, patch, key, v)
+                s.boundaries[field][patch][key] = np.array(v)
+        return source
+
+
 class PrimalMap(object):
-    """The primal Function's call contract (density.py:101-105): positional
-    [rho, rhoU, rhoE, dt] + mesh tensors + mesh scalars + sources + boundary
-    values; returns [rho', rhoU', rhoE', dtc (1,1), obj (1,1)]."""
+    """The primal Function's call contract (density.py:101-105, called at
+    solver.py:312-323): positional [rho (nIC,1), rhoU (nIC,3), rhoE (nIC,1),
+    dt (1,1)] + mesh.getTensor() + mesh.getScalar() + the 3 source arrays +
+    the boundary-value arrays (bc_inputs order); returns [rho', rhoU',
+    rhoE', dtc (1,1), obj (1,1)] as host arrays.  The source and boundary
+    values passed are the ones used (PositionalInputs).  Called with the
+    state and dt alone, it uses the solver's own source and boundary
+    values.  Options: return_reusable, replace_reusable (accepted; the
+    state always comes back)."""
+
+    OPTIONS = ('return_reusable', 'replace_reusable')
 
     def __init__(self, solver):
         self.solver = solver
+        self.inputs = PositionalInputs(solver)
 
-    def __call__(self, rho, rhoU, rhoE, dt, *rest, **options):
+    def state(self, rho, rhoU, rhoE):
         s = self.solver
         torch = s._torch()
         ctx = s.ctx
-        td = ctx.tdtype
         nIC = ctx.nIC
-        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=td).to('cuda')
-        u = ctx.pack(dev(np.reshape(rho, (nIC, 1))), dev(np.reshape(rhoU, (nIC, 3))), dev(np.reshape(rhoE, (nIC, 1))))
+        dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=ctx.tdtype).to('cuda')
+        return ctx.pack(dev(np.reshape(rho, (nIC, 1))), dev(np.reshape(rhoU, (nIC, 3))),
+                        dev(np.reshape(rhoE, (nIC, 1))))
+
+    def __call__(self, rho, rhoU, rhoE, dt, *rest, **options):
+        for k in options:
+            if k not in self.OPTIONS:
+                raise TypeError('unknown option %r' % k)
+        s = self.solver
+        torch = s._torch()
+        ctx = s.ctx
+        source = self.inputs.consume(rest) if rest else s.source
+        u = self.state(rho, rhoU, rhoE)
         dtc = torch.zeros(1, dtype=torch.float64, device='cuda')
         obj = torch.zeros(1, dtype=torch.float64, device='cuda')
-        ctx.primal_step(u, float(np.asarray(dt).reshape(-1)[0]), source=s.source, dtc_out=dtc, obj_out=obj)
+        s.step(u, float(np.asarray(dt).reshape(-1)[0]), source, dtc, obj)
         r, rU, rE = ctx.unpack(u)
         f = lambda t: t.cpu().numpy()
         return [f(r), f(rU), f(rE), dtc.cpu().numpy().reshape(1, 1), obj.cpu().numpy().reshape(1, 1)]

@@ -169,5 +169,21 @@ class Mesh(object):
             out.extend([self.boundary[pid][a] for a in Mesh.BCFields])
         return out
 
+    def tensor(self, name, device='cuda'):
+        """A mesh array as a float64 (int64 for index arrays) torch tensor on
+        `device`, cached: for objectives written as torch functions of the
+        fields (the role of `mesh.symMesh` arrays in the reference's case
+        files, templates/cylinder_test.py:9-19)."""
+        import torch
+        cache = self.__dict__.setdefault('_tensors', {})
+        key = (name, str(device))
+        if key not in cache:
+            a = np.asarray(getattr(self, name))
+            dt = torch.int64 if np.issubdtype(a.dtype, np.integer) else torch.float64
+            cache[key] = torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(device)
+        return cache[key]
+
     def copy(self):
-        return copy.deepcopy(self)
+        c = copy.copy(self)
+        c.__dict__ = {k: copy.deepcopy(v) for k, v in self.__dict__.items() if k != '_tensors'}
+        return c

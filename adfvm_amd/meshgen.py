@@ -245,6 +245,23 @@ def box(n, lengths=(1., 1., 1.), origin=(0., 0., 0.), patches=None, x_coords=Non
     return _structured(coords, (nx, ny, nz), False, patches)
 
 
+def transform(mesh, fn):
+    """The same mesh with every point moved by fn((nP,3) -> (nP,3)): an
+    affine map keeps faces planar and turns a Cartesian box into
+    parallelepiped cells whose centre-to-centre directions are not the face
+    normals (a non-orthogonal mesh)."""
+    out = mesh.copy()
+    out.points = np.ascontiguousarray(fn(out.points), np.float64)
+    return out
+
+
+def sheared_box(n, lengths=(1., 1., 1.), shear=((1., 0.3, 0.2), (0., 1., 0.25), (0., 0., 1.)),
+                patches=None, x_coords=None):
+    """Cartesian box mapped by the matrix `shear` (points x -> shear @ x)."""
+    S = np.asarray(shear, np.float64)
+    return transform(box(n, lengths, patches=patches, x_coords=x_coords), lambda p: p @ S.T)
+
+
 def periodic_box(n, L=2 * np.pi, origin=(0., 0., 0.)):
     """Fully periodic cube: three cyclic patch pairs (like the convection
     case's blockMeshDict, `cases/convection/constant/polyMesh/blockMeshDict:41-90`)."""

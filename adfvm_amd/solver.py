@@ -133,7 +133,7 @@ class Solver(object):
             if mode == 'forward':
                 solutions.append(self.u.clone())
             tape = tapes[timeIndex - startIndex] if tapes is not None else None
-            self.ctx.primal_step(self.u, dt, source=self.source, dtc_out=dtc, obj_out=obj, tape=tape)
+            self.step(self.u, dt, self.source, dtc, obj, tape)
             timeSteps.append([t, dt])
             timeIndex += 1
             t = updateTime(t, dt)
@@ -165,6 +165,29 @@ class Solver(object):
             solutions.append(self.u.clone())     # nSteps+1 states, as solver.py:234-241,376-382
             return solutions
         return result + acc.item()
+
+    def step(self, u, dt, source, dtc_out, obj_out, tape=None):
+        """One time step of the device state u in place, dtc and the
+        objective (FP64, on the first intermediate stage) into the device
+        scalars dtc_out / obj_out.  A case-file objective is evaluated by
+        torch on the objective stage, which the step's tape (a scratch one
+        if none is given) or u_n holds."""
+        from .objectives import PythonObjective
+        pyobj = self.objective if isinstance(getattr(self, 'objective', None), PythonObjective) else None
+        if pyobj is None:
+            self.ctx.primal_step(u, dt, source=source, dtc_out=dtc_out, obj_out=obj_out, tape=tape)
+            return
+        tape = tape if tape is not None else self._scratch_tape()
+        u_n = u.clone() if self.ctx.n_stages == 1 else None
+        self.ctx.primal_step(u, dt, source=source, dtc_out=dtc_out, tape=tape)
+        if obj_out is not None:
+            obj_out.copy_(pyobj.value(self.ctx, self, self.ctx.stage_state(u_n, tape)))
+
+    def _scratch_tape(self):
+        t = getattr(self, '_objTape', None)
+        if t is None or t.dtype != self.ctx.tdtype:
+            t = self._objTape = self.ctx.new_tapes(1)[0]
+        return t
 
     def _perturb(self, perturbation, t, revert):
         parameters, perturb = perturbation

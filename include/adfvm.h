@@ -191,6 +191,10 @@ int adfvm_ctx_create(const adfvm_solver_desc* desc, const adfvm_physics* phys, a
 void adfvm_ctx_destroy(adfvm_ctx* ctx);
 const char* adfvm_last_error(const adfvm_ctx* ctx);
 int adfvm_ctx_precision(const adfvm_ctx* ctx);
+/* SHA-256 (hex) of the sources the library was built from; the Python
+ * loader refuses a library whose hash differs from the sources beside it.
+ * No counterpart in the reference (adpy recompiles generated code). */
+const char* adfvm_source_hash(void);
 
 /* Cell segments: an ensemble of independent meshes concatenated into one
  * context, mesh k owning cells [cellStart[k], cellStart[k+1]).  The
@@ -249,6 +253,19 @@ int64_t adfvm_tape_bytes(const adfvm_ctx* ctx);
 int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* source,
                        void* ua, double obja, void* source_grad, void* bc_grad, const void* tape,
                        void* stream);
+
+/* adfvm_adjoint_step for an objective the caller evaluates itself: obj_grad
+ * (device [5][nIC]) is dJ_n/du at the step's objective stage (the first
+ * intermediate state, density.py:412-413 -- the stage-1 state a tape holds
+ * first, or u_n for a one-stage integrator), added with weight obja to the
+ * adjoint there, besides the context's own objective (if any).  The
+ * reference's case-file objective(fields, solver) is differentiated by adpy
+ * as part of the primal Function (density.py:354-357, apps/adjoint.py:101);
+ * here the caller differentiates its objective and chains the primitive
+ * fields to the state with adfvm_primitive_ghosts_vjp. */
+int adfvm_adjoint_step_cotangent(adfvm_ctx* ctx, const void* u_n, double dt, const void* source, void* ua,
+                                 double obja, const void* obj_grad, void* source_grad, void* bc_grad,
+                                 const void* tape, void* stream);
 
 /* Adjoint-viscosity stabilisation of the adjoint state after an adjoint step
  * (the viscous map of apps/adjoint.py:127-140 with computeAdjointViscosity
@@ -315,6 +332,17 @@ int adfvm_objective(adfvm_ctx* ctx, const void* u, double* obj_out, void* stream
 /* Primitive fields with ghost cells, the 'init' Function's boundary update
  * (density.py:64-78): prim device [5][nCells] rows Ux,Uy,Uz,T,p. */
 int adfvm_primitive_ghosts(adfvm_ctx* ctx, const void* u, void* prim, void* stream);
+
+/* Reverse of adfvm_primitive_ghosts: ubar (device [5][nIC], overwritten) =
+ * (d prim/d u)^T prim_bar for prim_bar device [5][nCells] (ghost rows
+ * included): ghost rows chain to their owner through the boundary rule, a
+ * cyclic ghost row to the paired cell it copies.  bc_grad (device
+ * [5][nBoundaryFaces], nullable): the ghost rows' dependence on the
+ * boundary values is ADDED to it (rows as in adfvm_adjoint_step).  Lets a
+ * caller's objective of the primitive fields (templates/cylinder_test.py:
+ * 9-37, density.py:354-357) be chained to the conservative state. */
+int adfvm_primitive_ghosts_vjp(adfvm_ctx* ctx, const void* u, const void* prim_bar, void* ubar, void* bc_grad,
+                               void* stream);
 
 /* Gauss gradient of an arbitrary cell field given on all nCells (ghost values
  * included, no boundary update), the reference unit-test operator

@@ -34,9 +34,24 @@ How each function is pinned (tests/test_oracle.py unless named):
                            (tests/golden/shocktube_exact_output.txt, the
                            reference's cases/shockTube/output)
   objective_value          drag and kinetic energy in closed form
-Parity unpinned: stabilise() (the semantics of adpy's `stabilise`, whose
-source is absent; only its use at riemann.py:52 is known) and the bitwise
-order of the reference's face->cell sums.
+  face_gradient            its defining property on a sheared (non-orthogonal)
+                           mesh: the component along D is the difference
+                           quotient, the part normal to D is unchanged
+  viscous_flux             (also) the Newtonian stress with Stokes' hypothesis
+                           mu (G + G^T - 2/3 tr(G) I) n for a linear velocity
+                           of arbitrary gradient G (dilatation included) on a
+                           sheared mesh; the 4/3 mu a normal stress of a pure
+                           dilatation
+  stabilise                immaterial: the residual and its autograd VJP are
+                           unchanged (<= 1e-15) under the forms that keep
+                           stabilise(0) > 0, x + SMALL (= x + sign(x) SMALL
+                           with sign(0) = 1) and max(x, SMALL), on TGV, Sod,
+                           mixed-boundary and uniform states
+  symmetriser /            A0 = dU/dv, the inverse of the Jacobian of the
+  adjoint_energy           entropy variables v(U) (autograd); w = e_1 gives
+                           sum rho V; a uniform state in closed form
+Parity unpinned: the bitwise order of the reference's face->cell sums
+(adpy's Tensor.collate), which this restatement does not claim.
 
 Inputs: an oracle mesh (oracle.mesh_oracle.build) and a case spec:
   bcs[patch][field] = {'type': 'cyclic'|'zeroGradient'|'empty'|'inletOutlet'|
@@ -279,7 +294,9 @@ class Case(object):
     # --------------------------------------------------- Riemann solvers --
     def stabilise(self, x):
         """eps.stabilise(SMALL), riemann.py:52; taken as x + sign(x)*SMALL
-        (x >= 0 here).  adpy is absent, so this is parity unpinned."""
+        (x >= 0 here).  adpy's definition is absent; every form that keeps
+        stabilise(0) > 0 gives the same residual and VJP to <= 1e-15
+        (test_stabilise_form_is_immaterial)."""
         return torch.where(x < 0, x - self.small, x + self.small)
 
     def euler_roe(self, pL, pR, TL, TR, UL, UR, rhoL, rhoR, rhoUL, rhoUR, rhoEL, rhoER, n):
@@ -337,17 +354,24 @@ class Case(object):
         return self.euler_roe if name == 'roe' else self.euler_lf
 
     # ------------------------------------------------------------ fluxes --
+    def face_gradient(self, gradF, phiL, phiR, faces):
+        """Non-orthogonal face gradient of density.py:197-200: the averaged
+        gradient gradF (nf,k,3) with its component along the unit
+        centre-to-centre direction D replaced by the difference quotient
+        snGrad = (phiR - phiL)/delta (op.snGrad, op.py:66-67):
+            gradF + snGrad D - (gradF . D) D."""
+        D = self.t['deltasUnit'][faces][:, None, :]
+        dl = self.t['deltas'][faces][:, None]
+        sn = (phiR - phiL) / dl                              # (nf,k)
+        return gradF + sn[:, :, None] * D - (gradF * D).sum(-1)[:, :, None] * D
+
     def viscous_flux(self, TL, TR, UL, UR, TF, UF, gradTF, gradUF, faces):
         """adFVM/density.py:192-222 with op.snGrad (op.py:66-67)."""
         mu = self.mu(TF)
         kappa = mu * (self.Cp / self.Pr)                     # density.py:46
-        D = self.t['deltasUnit'][faces]
         N = self.t['normals'][faces]
-        dl = self.t['deltas'][faces][:, None]
-        gT = gradTF[:, 0]                                    # (nf,3)
-        gT = gT + ((TR - TL) / dl) * D - (gT * D).sum(-1, keepdim=True) * D
-        snU = (UR - UL) / dl                                 # (nf,3)
-        gU = gradUF + snU[:, :, None] * D[:, None, :] - (gradUF * D[:, None, :]).sum(-1)[:, :, None] * D[:, None, :]
+        gT = self.face_gradient(gradTF, TL, TR, faces)[:, 0]     # (nf,3)
+        gU = self.face_gradient(gradUF, UL, UR, faces)           # (nf,3,3)
         qF = kappa * (gT * N).sum(-1, keepdim=True)
         tmp2 = ((gU + gU.transpose(1, 2)) * N[:, None, :]).sum(-1)
         tmp3 = (gU[:, 0, 0] + gU[:, 1, 1] + gU[:, 2, 2])[:, None]
@@ -476,6 +500,37 @@ class Case(object):
             mungUx = self.mu(T0) * (U0 - U0i) / self.t['deltas'][faces][:, None]
             return ((p0 * nx - mungUx) * self.t['areas'][faces][:, None]).sum()
         raise NotImplementedError(kind)
+
+    def symmetriser(self, rho, rhoU, rhoE):
+        """The 5x5 matrix A0 per cell of computeEnergy, cmesh.cpp:384-418
+        (nIC,5,5), in conservative order rho, rhoU, rhoE: rows
+        [rho, rho u, rhoE], [rho u, rho u u^T + p I, rho H u],
+        [rhoE, rho H u^T, rho H^2 - gamma p^2/(rho (gamma-1))]."""
+        g = self.gamma
+        r = rho.reshape(-1)
+        u = rhoU.reshape(-1, 3) / r[:, None]
+        E = rhoE.reshape(-1)
+        q2 = (u * u).sum(-1)
+        p = (E - r * q2 / 2) * (g - 1)
+        H = g * p / (r * (g - 1)) + q2 / 2
+        n = r.shape[0]
+        A = torch.zeros(n, 5, 5, dtype=r.dtype)
+        A[:, 0, 0] = r
+        A[:, 0, 1:4] = A[:, 1:4, 0] = r[:, None] * u
+        A[:, 0, 4] = A[:, 4, 0] = E
+        A[:, 1:4, 1:4] = r[:, None, None] * u[:, :, None] * u[:, None, :] + p[:, None, None] * torch.eye(3, dtype=r.dtype)
+        A[:, 1:4, 4] = A[:, 4, 1:4] = (r * H)[:, None] * u
+        A[:, 4, 4] = r * H * H - g * p * p / (r * (g - 1))
+        return A
+
+    def adjoint_energy(self, rhoa, rhoUa, rhoEa, rho, rhoU, rhoE):
+        """computeEnergy, cmesh.cpp:342-425: sum_c w^T A0 w V_c with
+        w = (rhoa, rhoUa, rhoEa)/V_c, the adjoint fields kept times the
+        cell volume (apps/adjoint.py:185-189)."""
+        V = self.t['volumes']
+        w = torch.cat([rhoa.reshape(-1, 1), rhoUa.reshape(-1, 3), rhoEa.reshape(-1, 1)], 1) / V[:, None]
+        A = self.symmetriser(rho, rhoU, rhoE)
+        return ((w[:, :, None] * A * w[:, None, :]).sum((1, 2)) * V).sum()
 
     # --------------------------------------------------------------- step --
     def coefficients(self):

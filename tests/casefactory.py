@@ -112,13 +112,20 @@ def sod_state(case, split=0.5):
     return state_from_prims(U, T, p)
 
 
-def mixed_box(seed=0, cbc='CBC_UPT', riemann='roe', boundaryRiemann='roe', sutherland=False, objective=None):
+def mixed_box(seed=0, cbc='CBC_UPT', riemann='roe', boundaryRiemann='roe', sutherland=False, objective=None,
+              shear=None):
     """Every ghost rule at once on a small random-renumbered box: fixedValue
-    inlet, characteristic outlet, slip and no-slip walls, empty sides."""
+    inlet, characteristic outlet, slip and no-slip walls, empty sides.
+    shear: a 3x3 matrix mapping the box's points (a non-orthogonal mesh)."""
     patches = {'x0': ('inlet', 'patch'), 'x1': ('outlet', 'patch'),
                'y0': ('bottom', 'symmetryPlane'), 'y1': ('top', 'patch'),
                'z0': ('sides', 'empty'), 'z1': ('sides', 'empty')}
-    md = meshgen.permute_and_rcm(meshgen.box((6, 5, 3), (1.2, 1.0, 0.6), patches=patches), seed=seed)
+    if shear is None:
+        base = meshgen.box((6, 5, 3), (1.2, 1.0, 0.6), patches=patches)
+    else:
+        base = meshgen.sheared_box((6, 5, 3), (1.2, 1.0, 0.6), shear, patches=patches,
+                                   x_coords=[0., .15, .35, .6, .8, 1.0, 1.2])
+    md = meshgen.permute_and_rcm(base, seed=seed)
     rng = np.random.default_rng(seed)
     n_in, n_out, n_top = 5 * 3, 5 * 3, 6 * 3
     bcs = {
@@ -211,3 +218,31 @@ def bc_value_tensors(case, params):
         bcs[patch][field][key] = t
         ts[(field, patch, key)] = t
     return bcs, ts
+
+
+def rcf(case, u, precision, objective='case', **config_kw):
+    """An RCF solver (adfvm_amd.density) on a CaseDef's mesh, boundary spec
+    and physics, its state set to u (5, nIC): the host layer the positional
+    maps and case-file objectives sit on.  objective: 'case' takes the
+    CaseDef's, else an objective for RCF(objective=...)."""
+    from adfvm_amd import config, objectives
+    from adfvm_amd.density import RCF
+    config.set_precision(precision)
+    if objective == 'case':
+        objective = None
+        if case.objective:
+            objective = objectives.kinetic_energy() if case.objective[0] == 'kinetic_energy' else \
+                objectives.drag(case.objective[1])
+    names = {'roe': 'eulerRoe', 'lf': 'eulerLaxFriedrichs'}
+    s = RCF(case.md, mu=None if case.sutherland else case.mu0, riemannSolver=names[case.riemann],
+            boundaryRiemannSolver=names[case.boundaryRiemann], objective=objective,
+            faceReconstructor='secondOrderLimited' if case.limiter else 'secondOrder', **config_kw)
+    bnd = {f: {} for f in ('U', 'T', 'p')}
+    for patch, spec in case.bcs.items():
+        for f, fs in spec.items():
+            bnd[f][patch] = dict(fs)
+    s.boundaries = bnd
+    s.compile()
+    import torch
+    s.u = torch.as_tensor(u.astype(precision), device='cuda').contiguous()
+    return s

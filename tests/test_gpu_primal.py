@@ -29,6 +29,8 @@ def oracle_fields(case, u):
     return oc, grad, R, dtc[:, 0].numpy(), prim
 
 
+SHEAR = ((1., 0.3, 0.2), (0., 1., 0.25), (0., 0., 1.))
+
 CASES = {
     'tgv6': lambda: (lambda c: (c[0], cf.tgv_state(*c)))(cf.tgv(6, seed=3)),
     'sod': lambda: (lambda c: (c, cf.sod_state(c)))(cf.sod(40)),
@@ -39,6 +41,11 @@ CASES = {
     # follows the owner), Sutherland viscosity
     'mixed_drag': lambda: (lambda c: (c, cf.smooth_state(c, 4)))(cf.mixed_box(4, 'CBC_UPT', sutherland=True,
                                                                               objective=('drag', 'top'))),
+    # a sheared (non-orthogonal) mesh: centre-to-centre directions differ
+    # from the face normals, so the snGrad correction is active
+    'mixed_skew': lambda: (lambda c: (c, cf.smooth_state(c, 6)))(cf.mixed_box(6, 'CBC_UPT', sutherland=True,
+                                                                              objective=('kinetic_energy',),
+                                                                              shear=SHEAR)),
 }
 
 
@@ -146,8 +153,28 @@ def test_dot_and_energy_fp64_accumulation():
     d = ctx.dot(a, b).item()
     ref = float((u.astype(np.float32).astype(np.float64) * np.float64(np.float32(1e-3))).sum())
     assert abs(d - ref) <= 1e-12 * abs(ref)
-    e = ctx.adjoint_energy(b, a).item()
-    assert np.isfinite(e) and e > 0
+
+
+@pytest.mark.parametrize('name', ['mixed_drag', 'mixed_skew', 'tgv6'])
+def test_adjoint_energy_against_oracle(name):
+    """adfvm_adjoint_energy (computeEnergy, cmesh.cpp:342-425) against the
+    oracle's restatement on a random adjoint field: FP64 relative error
+    <= 1e-12; the FP32 kernel (FP64 arithmetic on FP32 inputs and FP32 cell
+    volumes) against the oracle on the same FP32-rounded state and adjoint
+    <= 1e-6 (the volumes' rounding)."""
+    case, u = CASES[name]()
+    oc = case.oracle()
+    rng = np.random.default_rng(5)
+    ua = rng.normal(size=u.shape) * case.mesh.volumes.reshape(1, -1)
+    for prec in (np.float64, np.float32):
+        uu, aa = u.astype(prec).astype(np.float64), ua.astype(prec).astype(np.float64)
+        t = lambda a: torch.as_tensor(a, dtype=torch.float64)
+        want = float(oc.adjoint_energy(t(aa[0]), t(aa[1:4].T), t(aa[4]), t(uu[0:1].T), t(uu[1:4].T), t(uu[4:5].T)))
+        ctx = case.context(prec)
+        got = ctx.adjoint_energy(torch.as_tensor(ua.astype(prec), device='cuda').contiguous(),
+                                 torch.as_tensor(u.astype(prec), device='cuda').contiguous()).item()
+        print('%s %s energy %.15e vs %.15e' % (name, np.dtype(prec).name, got, want))
+        assert abs(got - want) <= (1e-12 if prec == np.float64 else 1e-6) * abs(want)
 
 
 def test_sod_1000_cells_200_steps_against_oracle():

@@ -65,7 +65,49 @@ def test_objectives():
     d = objectives.resolve(('drag', 'cylinder'))
     assert d.code == _lib.OBJ_DRAG and d.patch == 'cylinder'
     with pytest.raises(ValueError):
-        objectives.resolve(lambda fields, solver: 0)
+        objectives.resolve(42)
+    assert isinstance(objectives.resolve(lambda fields, solver: 0), objectives.PythonObjective)
+
+
+def test_case_file_objective_host_flow():
+    """PythonObjective: fields are the rows of the primitive array as
+    (U (nC,3), T (nC,1), p (nC,1)) float64 tensors; the cotangent passed to
+    the primitive-ghost VJP is weight * dJ/dprim in the context precision.
+    (A stub context stands in for the device.)"""
+    import torch
+    nC, nIC = 6, 4
+    prim = torch.arange(5 * nC, dtype=torch.float32).reshape(5, nC) + 1
+    seen = {}
+
+    class Ctx(object):
+        tdtype = torch.float32
+
+        def primitive_ghosts(self, u):
+            return prim.clone()
+
+        def primitive_ghosts_vjp(self, u, pbar, bc_grad=None):
+            seen['pbar'] = pbar
+            return pbar[:, :nIC] * 1
+
+    class Solver(object):
+        def _torch(self):
+            return torch
+
+    def J(fields, solver):
+        U, T, p = fields
+        assert U.shape == (nC, 3) and T.shape == (nC, 1) and p.shape == (nC, 1) and U.dtype == torch.float64
+        return (p[:, 0] * T[:, 0]).sum() + (U[:, 1] ** 2).sum()
+    po = objectives.resolve(J)
+    v = po.value(Ctx(), Solver(), None)
+    P = prim.double()
+    assert float(v) == float((P[4] * P[3]).sum() + (P[1] ** 2).sum())
+    _, ub = po.cotangent(Ctx(), Solver(), None, weight=0.5)
+    want = torch.zeros(5, nC, dtype=torch.float64)
+    want[4], want[3], want[1] = P[3], P[4], 2 * P[1]
+    assert seen['pbar'].dtype == torch.float32
+    assert torch.equal(seen['pbar'], (0.5 * want).float())
+    with pytest.raises(TypeError):
+        objectives.resolve(lambda f, s: 1.0).value(Ctx(), Solver(), None)
 
 
 def test_solver_config():
@@ -92,3 +134,48 @@ def test_primitive_conservative_round_trip():
     U2, T2, p2 = s.primitive(*s.conservative(U, T, p))
     assert np.allclose(U2, U, rtol=1e-13, atol=1e-12)
     assert np.allclose(T2, T, rtol=1e-13) and np.allclose(p2, p, rtol=1e-13)
+
+
+def test_context_rejects_bad_arrays_before_any_launch():
+    """Every Context entry validates shape, dtype, contiguity and device
+    before it calls the library: a tensor in the wrong precision, the
+    (3k, nIC) gradient where (3k, nCells) is needed, operands of different
+    sizes or a host tensor raise ValueError and nothing is launched."""
+    import torch
+    from adfvm_amd.context import Context
+
+    calls = []
+
+    class Recorder(object):
+        def __getattr__(self, name):
+            def f(*a):
+                calls.append(name)
+                return 0
+            return f
+
+    class MeshStub(object):
+        nCells, nFaces, nInternalCells = 14, 40, 10
+    ctx = object.__new__(Context)
+    ctx.torch, ctx.lib, ctx.handle = torch, Recorder(), None
+    ctx.precision = np.dtype(np.float32)
+    ctx.mesh, ctx.nIC, ctx.nBF = MeshStub(), 10, 4
+    f32 = lambda *s: torch.zeros(*s, dtype=torch.float32)
+    bad = [
+        lambda: ctx.primal_step(torch.zeros(5, 10, dtype=torch.float64), 1e-3),     # wrong precision
+        lambda: ctx.primal_step(f32(5, 9), 1e-3),                                   # wrong shape
+        lambda: ctx.primal_step(f32(10, 5).T, 1e-3),                                # not contiguous
+        lambda: ctx.primal_step(f32(5, 10), 1e-3),                                  # host tensor
+        lambda: ctx.adjoint_step(f32(5, 10), 1e-3, f32(5, 10), bc_grad=f32(5, 3)),
+        lambda: ctx.interp_second_order(f32(2, 14), f32(6, 10), 0),                 # gradient without ghost rows
+        lambda: ctx.interp_second_order(f32(2, 14), f32(6, 14), 2),
+        lambda: ctx.grad_cell(f32(2, 10)),
+        lambda: ctx.dot(f32(5, 10), f32(5, 9)),
+        lambda: ctx.residual_vjp(f32(5, 10), f32(4, 10)),
+        lambda: ctx.adjoint_energy(f32(5, 10), torch.zeros(5, 10, dtype=torch.float64)),
+        lambda: ctx.objective(f32(5, 11)),
+        lambda: ctx.unpack(f32(5, 10)),
+    ]
+    for fn in bad:
+        with pytest.raises(ValueError):
+            fn()
+    assert calls == []
