@@ -13,7 +13,8 @@ Reported value: FP32 primal+adjoint cell-updates/s (nCells * K / time).
 Also reported: the same for FP64, primal-only rates, the FP32-vs-FP64
 relative L2 error per conserved variable after --l2-steps primal steps, the
 roofline of the dominant kernel (HIP events on the library's stream) and the
-CPU oracle timed on a bounded sample.
+CPU path (a C++/OpenMP FP64 step and adjoint step) timed on a bounded sample
+of the same workload.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 2]
 """
@@ -168,27 +169,48 @@ def triad_bandwidth(torch, n=1 << 28, reps=10):
     return 3 * n * 4 / s / 1e9
 
 
-def cpu_baseline(n=48, steps=20, dt=2e-4):
-    """The oracle (torch float64 on the host, autograd reverse) on a bounded
-    sample: TGV n^3, `steps` steps forward + reverse."""
-    import torch
-    from tests import casefactory as cf
-    case, params = cf.tgv(n, seed=0)
-    u0 = cf.tgv_state(case, params)
-    oc = case.oracle()
-    x = torch.tensor(u0, dtype=torch.float64, requires_grad=True)
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(mesh, mu0, u0, dt, steps=8):
+    """The CPU path timed on the host: the C++/OpenMP FP64 step and adjoint
+    step (oracle/cpu_step.cpp, built -O3 -fopenmp) on the same mesh, initial
+    state and dt as the GPU run, `steps` steps forward keeping their start
+    states, then `steps` adjoint steps (each re-running its stages, as the
+    reference's reverse sweep re-runs the primal), on every OpenMP thread
+    (OMP_NUM_THREADS)."""
+    from oracle import cpu_baseline as cb
+    cs = cb.CpuStep(mesh, mu0)
+    u = np.ascontiguousarray(u0, np.float64)
+    w = u.copy()
+    cs.step(w, dt)                                   # warm-up (first touch of the work arrays)
     t0 = time.perf_counter()
-    r, rU, rE = x[0:1].T, x[1:4].T, x[4:5].T
-    J = 0
+    states = []
     for _ in range(steps):
-        r, rU, rE, dtc, obj = oc.step(r, rU, rE, dt)
-        J = J + obj
-    (g,) = torch.autograd.grad(J, x)
-    t = time.perf_counter() - t0
-    nC = case.mesh.nInternalCells
-    return dict(value=nC * steps / t, unit='cell-updates/s primal+adjoint', cores=torch.get_num_threads(),
-                kind='port', sample='oracle (torch float64, autograd reverse) TGV %d^3, %d steps forward+reverse, '
-                                    '%.1f s' % (n, steps, t))
+        states.append(u.copy())
+        cs.step(u, dt)
+    t1 = time.perf_counter()
+    ua = np.zeros_like(u)
+    for k in reversed(range(steps)):
+        cs.adjoint_step(states[k], dt, ua, 1.0)
+    t2 = time.perf_counter()
+    nC = mesh.nInternalCells
+    threads = cs.threads()
+    return dict(value=nC * steps / (t2 - t0), unit='cell-updates/s primal+adjoint', cores=threads, kind='port',
+                sample='C++/OpenMP FP64 step + adjoint step (oracle/cpu_step.cpp), TGV %d cells (the bench mesh), '
+                       '%d steps forward + %d reverse at the bench dt, %d threads, %.1f s'
+                       % (nC, steps, steps, threads, t2 - t0),
+                primal_value=nC * steps / (t1 - t0), precision='fp64', cpu_model=cpu_model(),
+                os_cpu_count=os.cpu_count(), omp_num_threads=os.environ.get('OMP_NUM_THREADS'),
+                finite=bool(np.isfinite(ua).all() and np.isfinite(u).all()))
 
 
 def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
@@ -465,7 +487,8 @@ def main():
     if l2 is not None:
         out.update({'l2_rel_err_fp32_vs_fp64': l2, 'l2_steps': args.l2_steps})
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline()
+        out['cpu_baseline'] = cpu_baseline(m, case.mu0, u0, dt)
+        print('[bench] cpu baseline %s' % out['cpu_baseline'], file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 
