@@ -36,6 +36,8 @@ class GraphSweep(object):
         self.J = torch.zeros(nseg, dtype=torch.float64, device='cuda')
         self.bc_grad = ctx.bc_state() if bc_grad else None
         self._fwd = self._rev = None
+        self.keep_final = False          # set to keep the state after the forward sweep (u_final)
+        self.u_final = None
 
     def _forward_segment(self):
         ctx, u, dt = self.ctx, self.u, self.dt
@@ -91,6 +93,8 @@ class GraphSweep(object):
         for s in range(n_steps // self.interval):
             checkpoints.append(u.clone())
             run_fwd()
+        if self.keep_final:
+            self.u_final = u.clone()
         # reverse sweep, last segment first
         while checkpoints:
             u.copy_(checkpoints.pop())
