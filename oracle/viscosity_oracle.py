@@ -10,10 +10,12 @@ How each function is pinned (tests/test_oracle.py):
   gradients        exact for linear U, p on a Cartesian box whose fixedValue
                    ghosts hold the face values; divU = trace(gradU)
   matrix           closed forms: the uniform-state matrix (only the gradrho
-                   term of postpro.py:553 survives) and the linear-field
-                   velocity block; symmetry.  Parity unpinned: the entries
-                   coupling gradc and gradp into rows/columns 1-4 beyond
-                   those forms (checked only as HIP = oracle)
+                   term of postpro.py:553 survives), the linear-field
+                   velocity block, and every entry coupling gradp or gradc
+                   into rows/columns 0 and 4 on states where only gradp
+                   (linear p, uniform T) or only gradc (uniform p, c linear)
+                   is non-zero, with the largest eigenvalue of each such
+                   matrix in closed form; symmetry
   max_eigenvalue   numpy.linalg.eigvalsh (LAPACK), the routine the
                    reference calls (dsyev/ssyev, scaling.cpp:107-127)
   heat_coefficients / jacobi

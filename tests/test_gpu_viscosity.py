@@ -63,3 +63,27 @@ def test_adjoint_viscosity_bitwise_repeatable_and_no_iterations():
     a = ua.clone()
     ctx.adjoint_viscosity(ud, a, 1e-5, 'turkel', 1e-3, iterations=0)
     assert np.allclose(a.cpu().numpy(), ua.cpu().numpy(), rtol=1e-6)
+
+
+@pytest.mark.parametrize('kind', ['p', 'c'])
+@pytest.mark.parametrize('vtype', ['abarbanel', 'turkel'])
+def test_adjoint_viscosity_pressure_and_sound_speed_states(kind, vtype):
+    """The states on which the oracle's gradp and gradc couplings are pinned
+    entry by entry (only gradp, or only gradc, non-zero; U = 0, fixedValue
+    ghosts): HIP's scaled viscosity per cell against the oracle's."""
+    from oracle import viscosity_oracle as vo
+    from tests.test_oracle import _visc_state_box
+    _, oc, u, _, _, _, _, (md, bcs) = _visc_state_box(kind)
+    case = cf.CaseDef(md, bcs, 1e-5)
+    scaling = 2.5
+    gi = [torch.as_tensor(a, dtype=torch.float64) for a in cf.ghost_init(case)]
+    ref = vo.scaled_viscosity(oc, vtype, u, scaling, ghostInit=gi)
+    for prec, tol in ((np.float64, 1e-10), (np.float32, 1e-4)):
+        ctx = case.context(prec)
+        ud = torch.as_tensor(u.astype(prec), device='cuda').contiguous()
+        uad = torch.zeros_like(ud)
+        m2 = torch.zeros(ctx.nIC, dtype=ctx.tdtype, device='cuda')
+        ctx.adjoint_viscosity(ud, uad, 1e-5, vtype, scaling, iterations=0, m2norm_out=m2)
+        err = rel_l2(m2.cpu().numpy(), ref)
+        print('%s-only %s %s: scaled viscosity rel-L2 %.2e' % (kind, vtype, np.dtype(prec).name, err))
+        assert err < tol, err

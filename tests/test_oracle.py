@@ -523,6 +523,104 @@ def test_viscosity_matrix_closed_forms():
     assert np.array_equal(MS, np.swapaxes(MS, -1, -2))
 
 
+def _visc_state_box(kind):
+    """U = 0 on a permuted Cartesian box with fixedValue ghosts holding the
+    face values; kind 'p': p = p0 + b.x linear and T uniform (only gradp is
+    non-zero); kind 'c': p uniform and T = c^2/(g R) with c = c0 + k.x
+    linear (only gradc is non-zero).  Returns the oracle pieces, the cell
+    values of p, T, c and the exact gradient vector."""
+    from oracle import viscosity_oracle as vo
+    md = meshgen.permute_and_rcm(meshgen.box((4, 3, 3), (1.0, 0.9, 0.6)), seed=5)
+    om = mesh_oracle.build(md)
+    rng = np.random.default_rng(9)
+    vec = rng.normal(size=3)
+    b0 = om.boundary['walls']
+    fc = om.faceCentres[b0['startFace']:b0['startFace'] + b0['nFaces']]
+    x = om.cellCentres[:om.nInternalCells]
+    if kind == 'p':
+        vec *= 500.
+        pf, Tf = 1e5 + fc @ vec, np.full(len(fc), 300.)
+        p, T = 1e5 + x @ vec, np.full(len(x), 300.)
+    else:
+        vec *= 20.
+        cfun = lambda y: 340. + y @ vec
+        pf, Tf = np.full(len(fc), 1e5), cfun(fc) ** 2 / (G * R)
+        p, T = np.full(len(x), 1e5), cfun(x) ** 2 / (G * R)
+    bcs = {'walls': {'U': {'type': 'fixedValue', 'value': np.zeros((len(fc), 3))},
+                     'T': {'type': 'fixedValue', 'value': Tf[:, None]},
+                     'p': {'type': 'fixedValue', 'value': pf[:, None]}}}
+    c = fvm_oracle.Case(om, bcs, G, CP, 0.7, lambda T: T * 0 + 1e-5)
+    u = np.concatenate([a.T for a in cons(np.zeros((len(x), 3)), T[:, None], p[:, None])])
+    return vo, c, u, p, T, np.sqrt(G * R * T), vec, (md, bcs)
+
+
+def _arrow(alpha, beta):
+    """Symmetric 5x5 matrices whose only non-zero entries are (0, 1+k) =
+    (1+k, 0) = alpha_k and (4, 1+k) = (1+k, 4) = beta_k, and their largest
+    eigenvalue in closed form: for v = (x, y, z), Mv = lam v gives
+    lam^2 y = alpha (alpha.y) + beta (beta.y), so lam^2 is the largest
+    eigenvalue of the Gram matrix of alpha and beta."""
+    n = len(alpha)
+    M = np.zeros((n, 5, 5))
+    M[:, 0, 1:4] = M[:, 1:4, 0] = alpha
+    M[:, 4, 1:4] = M[:, 1:4, 4] = beta
+    aa, bb, ab = (alpha * alpha).sum(1), (beta * beta).sum(1), (alpha * beta).sum(1)
+    lam2 = (aa + bb) / 2 + np.sqrt(((aa - bb) / 2) ** 2 + ab * ab)
+    return M, np.sqrt(lam2)
+
+
+@pytest.mark.parametrize('kind', ['p', 'c'])
+def test_viscosity_matrix_pressure_and_sound_speed_couplings(kind):
+    """The entries coupling gradp and gradc into rows/columns 0..4
+    (postpro.py:542-621) on states where only one of them is non-zero, each
+    entry against its value from the exact gradient and the cell values:
+      gradp only (U = 0, T uniform):  gradrho = g (gradp - c p)/c^2 per
+        component; abarbanel MS[0,k] = -(b/rho) gradrho_k/2,
+        MS[k,4] = -a gradp_k/(4p); turkel MS[0,k] = -(gradp_k/(rho c) +
+        (g-1) gradp_k/(2 rho c))/2, MS[k,4] = -(gradp_k pref/(2 g p rho Uref)
+        + (gradp_k - c^2 gradrho_k) Uref/pref)/2;
+      gradc only (U = 0, p uniform):  gradrho = -g p/c per component;
+        abarbanel MS[0,k] = gradc_k/(2 sqrt(g)) - (b/rho) gradrho_k/2,
+        MS[k,4] = a'_k/2 - a'_k/(g-1) with a' = sqrt(g-1)/sqrt(g) gradc;
+        turkel MS[0,k] = gradc_k/2, MS[k,4] = -(g p c Uref/pref)/2.
+    The largest eigenvalue of each such matrix is also compared with its
+    closed form (LAPACK, the routine of scaling.cpp:107-127)."""
+    vo, c, u, p, T, cc, vec, _ = _visc_state_box(kind)
+    U_, T_, p_ = vo.primitive_with_ghosts(c, u)
+    gU, dU, gp, gc = vo.viscosity_gradients(c, U_, T_, p_)
+    assert np.abs(gU).max() < 1e-9 and np.abs(dU).max() < 1e-9
+    g = G
+    sg, g1 = np.sqrt(g), g - 1
+    sg1 = np.sqrt(g1)
+    rho = p / (R * T)
+    one = np.ones((len(p), 3))
+    if kind == 'p':
+        assert np.abs(gp - vec).max() < 1e-9 * np.abs(vec).max() and np.abs(gc).max() < 1e-9
+        Gp = one * vec
+        grho = g * (Gp - (cc * p)[:, None]) / (cc * cc)[:, None]
+        want = {
+            'abarbanel': (-(cc / sg / rho)[:, None] * grho / 2, -(sg1 * cc / sg)[:, None] * Gp / (4 * p)[:, None]),
+            'turkel': (-(Gp / (rho * cc)[:, None] + g1 * Gp / (2 * rho * cc)[:, None]) / 2,
+                       -(Gp * vo.PREF / (2 * g * p * rho * vo.UREF)[:, None]
+                         + (Gp - (cc * cc)[:, None] * grho) * vo.UREF / vo.PREF) / 2)}
+    else:
+        assert np.abs(gc - vec).max() < 1e-9 * np.abs(vec).max() and np.abs(gp).max() < 1e-6
+        Gc = one * vec
+        grho = one * (-g * p / cc)[:, None]
+        want = {
+            'abarbanel': (Gc / (2 * sg) - (cc / sg / rho)[:, None] * grho / 2,
+                          (sg1 / sg) * Gc / 2 - (sg1 / sg) * Gc / g1),
+            'turkel': (Gc / 2, -one * (g * p * cc * vo.UREF / vo.PREF)[:, None] / 2)}
+    for vtype, (alpha, beta) in want.items():
+        MS = vo.matrix(c, vtype, U_, T_, p_, gU, dU, gp, gc)
+        M_want, lam = _arrow(alpha, beta)
+        scale = np.abs(M_want).max()
+        err = np.abs(MS - M_want).max() / scale
+        print(kind, vtype, 'max entry error / max entry %.2e' % err)
+        assert err < 1e-9, (kind, vtype, err)
+        assert np.allclose(vo.max_eigenvalue(MS), lam, rtol=1e-9), (kind, vtype)
+
+
 def test_viscosity_heat_step_against_direct_solve():
     """The Jacobi sweeps converge to the direct solution of the implicit
     step (I + dt/V L) phi = b over internal faces; a uniform field is
