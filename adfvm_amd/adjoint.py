@@ -217,12 +217,13 @@ class Adjoint(object):
             source=None, sampleInterval=1, reportInterval=1, avgStart=0, runCheckpoints=2 ** 30, tapes=True,
             verbose=False, viscousInterval=1):
         """The reverse sweep of apps/adjoint.py:176-404 for the parameter
-        parameters[0]: 'source' or a boundary-condition value
-        ('BCs', field, patch, key) (apps/adjoint.py:104-118); returns the
-        (unaveraged) sensitivities, one per perturbation."""
+        parameters[0]: 'source', 'mesh' (the arrays of Mesh.gradFields) or a
+        boundary-condition value ('BCs', field, patch, key)
+        (apps/adjoint.py:104-118); returns the (unaveraged) sensitivities,
+        one per perturbation."""
         param = list(parameters)[0] if parameters else 'source'
-        if not (param == 'source' or (isinstance(param, tuple) and len(param) == 4 and param[0] == 'BCs')):
-            raise NotImplementedError('parameter %r (source and BC values are supported)' % (param,))
+        if not (param in ('source', 'mesh') or (isinstance(param, tuple) and len(param) == 4 and param[0] == 'BCs')):
+            raise NotImplementedError('parameter %r (source, mesh and BC values are supported)' % (param,))
         primal = self.primal
         ctx = primal.ctx
         torch = primal._torch()
@@ -238,15 +239,23 @@ class Adjoint(object):
         ua = self.ua
 
         perturbations = []
-        bgrad = bc_rows = bc_faces = None
-        if param != 'source':
+        bgrad = bc_rows = bc_faces = mgrad = None
+        if param == 'mesh':
+            mgrad = ctx.mesh_grad_state()
+        elif param != 'source':
             _, phi, patchID, key = param
             bc_rows, bc_faces = ctx.bc_param(phi, patchID, key)
             bgrad = ctx.bc_state()
         for p in perturb:
             values = p(None, mesh, 0)
             values = list(values) if isinstance(values, (list, tuple)) else [values]
-            if bgrad is None:
+            if mgrad is not None:        # the Mesh.gradFields arrays, in the gradient's layout
+                views = ctx.mesh_grad_arrays(mgrad)
+                if len(values) != len(views):
+                    raise ValueError('a mesh perturbation has %d arrays (Mesh.gradFields)' % len(views))
+                d = np.concatenate([np.broadcast_to(np.asarray(v, np.float64).reshape(-1, *tuple(w.shape[1:])),
+                                                    tuple(w.shape)).reshape(-1) for v, w in zip(values, views)])
+            elif bgrad is None:
                 vals = [np.asarray(v, np.float64).reshape(mesh.nInternalCells, -1) for v in values]
                 d = np.concatenate([v.T for v in vals])
             else:                        # one (nFaces, components) array, broadcast
@@ -303,13 +312,16 @@ class Adjoint(object):
                 states[k] = tps[k] = None
                 if r % sampleInterval == 0:
                     sens = []
-                    g = sgrad if bgrad is None else bgrad[bc_rows, bc_faces].contiguous()
+                    g = mgrad if mgrad is not None else (sgrad if bgrad is None else
+                                                         bgrad[bc_rows, bc_faces].contiguous())
                     for d in perturbations:
                         ctx.dot(g, d, out=dsum)
                         sens.append(dsum.item())
                     sgrad.zero_()
                     if bgrad is not None:
                         bgrad.zero_()
+                    if mgrad is not None:
+                        mgrad.zero_()
                     if (nSteps - (primalIndex + k)) > avgStart:
                         for i, sv in enumerate(sens):
                             result[i] += sv
@@ -325,6 +337,8 @@ class Adjoint(object):
                 np.savetxt(f, np.array(energyTimeSeries))
             sensTimeSeries, energyTimeSeries = [], []
             self.writeFields(t)
+        if mgrad is not None:
+            ctx.set_mesh_grad(None)
         self.finished = cp >= totalCheckpoints
         if self.finished:
             self.removeStatusFile()
@@ -337,8 +351,8 @@ class AdjointMap(object):
     + [rhoa (nIC,1), rhoUa (nIC,3), rhoEa (nIC,1), dtca (1,1), obja (1,1)]
     + [scaling (1,1)]; returns [rhoa', rhoUa', rhoEa'] + the parameter
     gradients: the 3 source arrays' for 'source' (apps/adjoint.py:104-106),
-    the value array's (nFaces, components) for ('BCs', field, patch, key)
-    (:111-118).  Adjoint fields are times volume, as the reference keeps
+    the 10 arrays of Mesh.gradFields for 'mesh' (:107-110), the value
+    array's (nFaces, components) for ('BCs', field, patch, key) (:111-118).  Adjoint fields are times volume, as the reference keeps
     them (apps/adjoint.py:185-189).  The parameter gradients accumulate over
     calls (the reference's static outputs): return_static returns them,
     zero_static zeroes them after (both default to True); with
@@ -357,14 +371,18 @@ class AdjointMap(object):
         self.pmap = PrimalMap(primal)
         self.parameter = parameter
         self.viscous = viscous
-        if not (parameter == 'source' or (isinstance(parameter, tuple) and len(parameter) == 4
-                                          and parameter[0] == 'BCs')):
-            raise NotImplementedError('parameter %r (source and BC values are supported)' % (parameter,))
-        if parameter != 'source':
+        if not (parameter in ('source', 'mesh') or (isinstance(parameter, tuple) and len(parameter) == 4
+                                                    and parameter[0] == 'BCs')):
+            raise NotImplementedError('parameter %r (source, mesh and BC values are supported)' % (parameter,))
+        if isinstance(parameter, tuple):
             _, phi, patchID, key = parameter
             self.rows, self.faces = self.ctx.bc_param(phi, patchID, key)
         self.sgrad = self.ctx.state(0.)
         self.bgrad = self.ctx.bc_state()
+        self.mgrad = None
+        if parameter == 'mesh':               # registered for this map's steps only
+            self.mgrad = self.ctx.mesh_grad_state()
+            self.ctx.set_mesh_grad(None)
 
     def __call__(self, *args, **options):
         for k in options:
@@ -392,22 +410,34 @@ class AdjointMap(object):
         if isinstance(primal.objective, PythonObjective):
             og = stage_cotangent(ctx, primal, u, dt, source, None, obja, self.bgrad)
             w = 1.0 if og is not None else obja
-        ctx.adjoint_step(u, dt, ua, obja=w, source=source, source_grad=self.sgrad, bc_grad=self.bgrad, obj_grad=og)
+        if self.mgrad is not None:
+            ctx.set_mesh_grad(self.mgrad)
+        try:
+            ctx.adjoint_step(u, dt, ua, obja=w, source=source, source_grad=self.sgrad, bc_grad=self.bgrad,
+                             obj_grad=og)
+        finally:
+            if self.mgrad is not None:
+                ctx.set_mesh_grad(None)
         if self.viscous and self.adjoint.viscosityType:
             ctx.adjoint_viscosity(u, ua, dt, self.adjoint.viscosityType,
                                   float(np.asarray(scaling).reshape(-1)[0]),
                                   iterations=self.adjoint.viscousIterations)
         r, rU, rE = ctx.unpack(ua)
         out = [r.cpu().numpy(), rU.cpu().numpy(), rE.cpu().numpy()]
+        n_static = {'source': 3, 'mesh': 10}.get(self.parameter, 1) if not isinstance(self.parameter, tuple) else 1
         if ret_static:
             if self.parameter == 'source':
                 g = self.sgrad.double().cpu().numpy()
                 out += [g[0:1].T.copy(), g[1:4].T.copy(), g[4:5].T.copy()]
+            elif self.parameter == 'mesh':
+                out += [a.double().cpu().numpy().copy() for a in ctx.mesh_grad_arrays(self.mgrad)]
             else:
                 out.append(self.bgrad[self.rows, self.faces].double().cpu().numpy().T.copy())
         else:
-            out += [None] * (3 if self.parameter == 'source' else 1)
+            out += [None] * n_static
         if zero_static:
             self.sgrad.zero_()
             self.bgrad.zero_()
+            if self.mgrad is not None:
+                self.mgrad.zero_()
         return out

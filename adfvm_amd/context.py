@@ -248,6 +248,42 @@ class Context(object):
                                               float(obja), _dptr(source_grad), _dptr(bc_grad), _dptr(tape),
                                               _stream(stream)))
 
+    # the reference's differentiable mesh arrays, in order (Mesh.gradFields, mesh.py:28-32)
+    MESH_GRAD_FIELDS = ('areas', 'volumesL', 'volumesR', 'weights', 'deltas', 'normals', 'deltasUnit',
+                        'linearWeights', 'quadraticWeights', 'volumes')
+
+    def mesh_grad_state(self):
+        """A zeroed device array for the mesh-geometry gradient (parameter
+        'mesh', apps/adjoint.py:107-110) and its registration: every later
+        adjoint step adds d(obja*J + <ua, u_{n+1}>)/d(mesh) to it, until
+        set_mesh_grad(None).  mesh_grad_arrays() splits it into the
+        reference's arrays."""
+        n = int(self.lib.adfvm_mesh_grad_size(self.handle))
+        g = self.torch.zeros(n, dtype=self.tdtype, device='cuda')
+        self.set_mesh_grad(g)
+        return g
+
+    def set_mesh_grad(self, g):
+        if g is not None:
+            self._check(g, (int(self.lib.adfvm_mesh_grad_size(self.handle)),), 'mesh_grad')
+        self._mesh_grad = g                  # keeps the array alive while the library holds it
+        self._chk(self.lib.adfvm_set_mesh_grad(self.handle, _dptr(g)))
+
+    def mesh_grad_arrays(self, g):
+        """Views of a mesh-gradient array as the reference's arrays: areas
+        (nF,1), volumesL (nF,1), volumesR (nIF,1), weights (nF,1), deltas
+        (nF,1), normals (nF,3), deltasUnit (nF,3), linearWeights (nF,2),
+        quadraticWeights (nF,2,3), volumes (nIC,1)."""
+        nF, nIF, nIC = self.mesh.nFaces, self.mesh.nInternalFaces, self.nIC
+        shapes = [(nF, 1), (nF, 1), (nIF, 1), (nF, 1), (nF, 1), (nF, 3), (nF, 3), (nF, 2), (nF, 2, 3), (nIC, 1)]
+        out, o = [], 0
+        for sh in shapes:
+            n = int(np.prod(sh))
+            out.append(g[o:o + n].reshape(sh))
+            o += n
+        assert o == g.numel()
+        return out
+
     @property
     def n_stages(self):
         return 1 if self.integrator == _lib.RK_EULER else 3

@@ -18,6 +18,7 @@ namespace adfvm {
 
 template <typename T> ADFVM_HD T fsqrt(T x) { return sqrt(x); }
 template <typename T> ADFVM_HD T fabs_(T x) { return x < T(0) ? -x : x; }
+template <typename T> ADFVM_HD T fmax_(T a, T b) { return a > b ? a : b; }
 
 template <typename T>
 struct Phys {
@@ -139,7 +140,7 @@ ADFVM_HD void lf_flux(const Phys<T>& ph, const T* wL, const T* qL, const T* wR, 
     T UnR = wR[0] * n[0] + wR[1] * n[1] + wR[2] * n[2];
     T cL = fsqrt(g * pL / rhoL), cR = fsqrt(g * pR / rhoR);
     T aL = fabs_(UnL) + cL, aR = fabs_(UnR) + cR;
-    T aF = aL > aR ? aL : aR;
+    T aF = fmax_(aL, aR);
     F[0] = T(0.5) * (rhoL * UnL + rhoR * UnR) - T(0.5) * aF * (rhoR - rhoL);
     for (int i = 0; i < 3; i++)
         F[1 + i] = T(0.5) * (qL[1 + i] * UnL + qR[1 + i] * UnR + (pL + pR) * n[i]) - T(0.5) * aF * (qR[1 + i] - qL[1 + i]);

@@ -24,6 +24,7 @@
 #include "adfvm.h"
 #include "fvm_math.hpp"
 #include "adjoint_math.hpp"
+#include "dual.hpp"
 
 using namespace adfvm;
 
@@ -115,7 +116,7 @@ template <bool ST, typename T> __device__ __forceinline__ void load_geo(const DM
     const long s = m.nF;
     g.A = ld<ST>(G + G_A * s + f);
     g.n[0] = ld<ST>(G + G_NX * s + f); g.n[1] = ld<ST>(G + G_NY * s + f); g.n[2] = ld<ST>(G + G_NZ * s + f);
-    g.w = ld<ST>(G + G_W * s + f);
+    g.w = T(0);           // the face weight enters gradients only (slot geometry), not the face flux
     g.delta = ld<ST>(G + G_DELTA * s + f);
     g.D[0] = ld<ST>(G + G_DX * s + f); g.D[1] = ld<ST>(G + G_DY * s + f); g.D[2] = ld<ST>(G + G_DZ * s + f);
     g.lw0 = ld<ST>(G + G_LW0 * s + f); g.lw1 = ld<ST>(G + G_LW1 * s + f);
@@ -179,22 +180,25 @@ __device__ __forceinline__ void load_grad(const T* __restrict__ grad, long nIC, 
 // primitive values w (Solver.boundary + characteristicBoundary,
 // solver.py:106-114, density.py:333-337, BCs.py): every rule reads only the
 // owner, the face normal and the patch's values, so the kernels that need a
-// ghost evaluate it where they use it.
-template <typename T>
-__device__ __forceinline__ void ghost_value(const DMesh<T>& m, const Phys<T>& ph, int b, const T* w, T* out) {
+// ghost evaluate it where they use it.  ghost_rule takes the normal from
+// nrm(n) (called only by the rules that use it) and computes in T, S or
+// Dual<S>; ghost_value reads the normal from the face geometry.
+template <typename T, typename S, typename NormalFn>
+__device__ __forceinline__ void ghost_rule(const DMesh<S>& m, const Phys<T>& ph, int b, const T* w,
+                                           NormalFn nrm, T* out) {
     const DPatch P = m.patches[m.bPatch[b]];
     const int f = m.nIF + b;
     const int k = f - P.start;
-    const T* V = m.bcval;
+    const S* V = m.bcval;
     if (P.cbc == ADFVM_CBC_UPT) {                    // BCs.py:152-161
-        for (int i = 0; i < 3; i++) out[i] = V[P.valU + 3 * k + i];
-        out[3] = V[P.valT + k];
-        out[4] = V[P.valp + k];
+        for (int i = 0; i < 3; i++) out[i] = T(V[P.valU + 3 * k + i]);
+        out[3] = T(V[P.valT + k]);
+        out[4] = T(V[P.valp + k]);
     } else if (P.cbc == ADFVM_CBC_TOTAL_PT) {        // BCs.py:164-184
         T d[3];
-        if (P.hasDir) { for (int i = 0; i < 3; i++) d[i] = V[P.valDir + 3 * k + i]; }
-        else { const long s = m.nF; d[0] = m.geo[G_NX * s + f]; d[1] = m.geo[G_NY * s + f]; d[2] = m.geo[G_NZ * s + f]; }
-        T Tt = V[P.valTt + k], pt = V[P.valpt + k];
+        if (P.hasDir) { for (int i = 0; i < 3; i++) d[i] = T(V[P.valDir + 3 * k + i]); }
+        else nrm(d);
+        T Tt = T(V[P.valTt + k]), pt = T(V[P.valpt + k]);
         T Un = w[0] * d[0] + w[1] * d[1] + w[2] * d[2];
         for (int i = 0; i < 3; i++) out[i] = Un * d[i];
         out[3] = Tt - T(0.5) * Un * Un / ph.Cp;
@@ -202,19 +206,28 @@ __device__ __forceinline__ void ghost_value(const DMesh<T>& m, const Phys<T>& ph
     } else {
         // U
         if (P.bcU == ADFVM_BC_FIXED_VALUE || P.bcU == ADFVM_BC_CALCULATED) {
-            for (int i = 0; i < 3; i++) out[i] = V[P.valU + 3 * k + i];
+            for (int i = 0; i < 3; i++) out[i] = T(V[P.valU + 3 * k + i]);
         } else {
             for (int i = 0; i < 3; i++) out[i] = w[i];
             if (P.bcU == ADFVM_BC_SYMMETRY) {        // BCs.py:125-133
-                const long s = m.nF;
-                T n0 = m.geo[G_NX * s + f], n1 = m.geo[G_NY * s + f], n2 = m.geo[G_NZ * s + f];
-                T un = out[0] * n0 + out[1] * n1 + out[2] * n2;
-                out[0] -= un * n0; out[1] -= un * n1; out[2] -= un * n2;
+                T n[3];
+                nrm(n);
+                T un = out[0] * n[0] + out[1] * n[1] + out[2] * n[2];
+                out[0] -= un * n[0]; out[1] -= un * n[1]; out[2] -= un * n[2];
             }
         }
-        out[3] = (P.bcT == ADFVM_BC_FIXED_VALUE || P.bcT == ADFVM_BC_CALCULATED) ? V[P.valT + k] : w[3];
-        out[4] = (P.bcp == ADFVM_BC_FIXED_VALUE || P.bcp == ADFVM_BC_CALCULATED) ? V[P.valp + k] : w[4];
+        out[3] = (P.bcT == ADFVM_BC_FIXED_VALUE || P.bcT == ADFVM_BC_CALCULATED) ? T(V[P.valT + k]) : w[3];
+        out[4] = (P.bcp == ADFVM_BC_FIXED_VALUE || P.bcp == ADFVM_BC_CALCULATED) ? T(V[P.valp + k]) : w[4];
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void ghost_value(const DMesh<T>& m, const Phys<T>& ph, int b, const T* w, T* out) {
+    const int f = m.nIF + b;
+    ghost_rule(m, ph, b, w, [&](T* n) {
+        const long s = m.nF;
+        n[0] = m.geo[G_NX * s + f]; n[1] = m.geo[G_NY * s + f]; n[2] = m.geo[G_NZ * s + f];
+    }, out);
 }
 
 // The ghost array (ghost primitives [5][nBF]), for the consumers that read
@@ -373,13 +386,14 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
 //   L_c = (1/V_c) sum_slots sign*(F A)_f     (op.div, op.py:12-20)
 //   lam_c = (1/V_c) sum_slots (lam A)_f      (op.absDiv, op.py:22-29)
 // MODE 0: uout = a0*u0 + ai*ui + (-beta*(L-S))*dt   (timestep.update, timestep.py:35-45)
+//         with a0 = 1 - ai when u0 is given (every such stage of the tables)
 // MODE 1: uout = L (the residual)
 // The last SSP-RK3 stage passes the step's state as both u0 and uout (each
 // thread reads u0[c] before writing uout[c] at the same index), so neither
 // is declared __restrict__.
 template <typename T, int MODE, int BS, bool ST>
 __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restrict__ fbuf,
-                                                      const T* __restrict__ ui, const T* u0, T a0,
+                                                      const T* __restrict__ ui, const T* u0,
                                                       T ai, T nbeta, T dt, const T* __restrict__ S,
                                                       T* uout, T* __restrict__ dtcCell,
                                                       double* __restrict__ dtcPartial) {
@@ -387,6 +401,13 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
     const long nIC = m.nIC, nF = m.nF;
     double dtcv = 0.0;
     if (c < m.nIC) {
+        // the cell's own rows first, so their loads overlap the face gathers
+        T v[5], v0[5], sv[5];
+        if (MODE == 0) {
+            for (int i = 0; i < 5; i++) v[i] = ld<ST>(ui + i * nIC + c);
+            for (int i = 0; i < 5; i++) v0[i] = u0 ? ld<ST>(u0 + i * nIC + c) : T(0);
+            for (int i = 0; i < 5; i++) sv[i] = S ? ld<ST>(S + i * nIC + c) : T(0);
+        }
         T L[5] = {0, 0, 0, 0, 0};
         T lamSum = 0;
         for (int j = 0; j < 6; j++) {
@@ -405,10 +426,17 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
             for (int i = 0; i < 5; i++) uout[i * nIC + c] = L[i];
         } else {
             for (int i = 0; i < 5; i++) {
-                T acc = ai * ld<ST>(ui + i * nIC + c);
-                if (u0) acc = a0 * ld<ST>(u0 + i * nIC + c) + acc;
-                T Li = S ? L[i] - ld<ST>(S + i * nIC + c) : L[i];
-                uout[i * nIC + c] = acc + (nbeta * Li) * dt;
+                const T Li = S ? L[i] - sv[i] : L[i];
+                if (u0) {
+                    // a0 = 1 - ai: uout = u0 + (ai*(ui - u0) - beta*(L-S)*dt), the
+                    // change formed at its own scale and added to u0 in one
+                    // rounding.  (ai*ui + a0*u0 rounds ai*ui on a finer grid than
+                    // the sum's; with ai = 2/3 that is biased by -1/9 ulp a step.)
+                    const T du = ai * (v[i] - v0[i]) + (nbeta * Li) * dt;
+                    uout[i * nIC + c] = v0[i] + du;
+                } else {
+                    uout[i * nIC + c] = ai * v[i] + (nbeta * Li) * dt;
+                }
             }
         }
         if (dtcCell) dtcCell[c] = lamSum;
@@ -691,6 +719,7 @@ struct adfvm_ctx {
     bool stream = false;
     int* fvjpBlk[4] = {nullptr, nullptr, nullptr, nullptr};
     int fvjpNBlk[4] = {0, 0, 0, 0};
+    void* meshGrad = nullptr;  // caller's mesh-geometry gradient array (adfvm_set_mesh_grad), or none
     void* fbuf = nullptr;      // [6][nF] face fluxes * area
     void* frec = nullptr;      // [nPlanes][nIC][AREC] face adjoint records, by cell slot
     // per-kernel event timing (adfvm_profile)
@@ -943,7 +972,17 @@ template <typename T> void launch_face_flux(adfvm_ctx* c, const T* u, const T* g
     });
 }
 
-struct Stage { double a0, ai, beta; };   // u_{i+1} = a0*u0 + ai*u_i - beta*(L-S)*dt
+// u_{i+1} = a0*u0 + ai*u_i - beta*(L-S)*dt.  Every stage of the tables
+// below with a0 != 0 has a0 + ai = 1, and stage_kernel evaluates it as
+// u0 + (ai*(u_i - u0) - beta*(L-S)*dt), so sum(V*u) is kept up to unbiased
+// rounding.  (Taken as a0*u0 + ai*u_i with the coefficients rounded apart,
+// SSP-RK3's 1/3 and 2/3 both round up in FP32, to a sum of 1 + 2^-25: the
+// state would grow by that factor every step.)  The adjoint uses a0 formed
+// as 1 - ai in the compute precision, the coefficient the primal applies.
+struct Stage {
+    double a0, ai, beta;
+    template <typename T> T a0_as() const { return a0 == 0.0 ? T(0) : T(1) - (T)ai; }
+};
 
 template <typename T, int MODE>
 void launch_stage(adfvm_ctx* c, const T* ui, const T* u0, const Stage& st, double dt, const T* S, T* out,
@@ -954,7 +993,7 @@ void launch_stage(adfvm_ctx* c, const T* ui, const T* u0, const Stage& st, doubl
             constexpr int BS = decltype(B)::value;
             constexpr bool ST = decltype(SP)::value;
             stage_kernel<T, MODE, BS, ST><<<grid_bs(c->nIC, BS), BS, 0, s>>>(
-                dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.a0, (T)st.ai, (T)(-st.beta), (T)dt, S, out, dtcCell,
+                dmesh<T>(c), P<T>(c->fbuf), ui, u0, (T)st.ai, (T)(-st.beta), (T)dt, S, out, dtcCell,
                 dtcPartial);
         });
     });
@@ -1462,5 +1501,6 @@ extern "C" int adfvm_unpack_state(adfvm_ctx* c, const void* u, void* rho, void* 
     return ADFVM_OK;
 }
 
+#include "mesh_adjoint.inc"
 #include "adjoint_kernels.inc"
 #include "viscosity.inc"

@@ -203,8 +203,34 @@ class Solver(object):
                 # host spec and the device copy
                 _, phi, patchID, key = param
                 self._perturb_bc(phi, patchID, key, sign * np.asarray(value, np.float64))
+            elif param == 'mesh':
+                # field += delta for each of Mesh.gradFields (solver.py:254-261)
+                self._perturb_mesh([sign * np.asarray(v, np.float64) for v in value])
             else:
-                raise NotImplementedError('perturbation of %r (source and BC values are supported)' % (param,))
+                raise NotImplementedError('perturbation of %r (source, mesh and BC values are supported)'
+                                          % (param,))
+
+    def _perturb_mesh(self, deltas):
+        """Adds deltas (the arrays of Mesh.gradFields, in order) to the mesh
+        and rebuilds the device context from it.  The kernels divide each
+        cell's face sum by its volume, so volumesL/volumesR perturbations
+        must be those of volumes at the faces' owners/neighbours."""
+        from .mesh import Mesh
+        m = self.mesh
+        if len(deltas) != len(Mesh.gradFields):
+            raise ValueError('a mesh perturbation has %d arrays (Mesh.gradFields)' % len(Mesh.gradFields))
+        d = dict(zip(Mesh.gradFields, deltas))
+        nIF = m.nInternalFaces
+        dV = d['volumes'].reshape(-1)
+        for name, want in (('volumesL', dV[m.owner]), ('volumesR', dV[m.neighbour[:nIF]])):
+            got = d[name].reshape(-1)
+            if got.shape != want.shape or not np.allclose(got, want, rtol=1e-12, atol=0):
+                raise NotImplementedError('%s perturbed apart from volumes: the step divides by cell volumes'
+                                          % name)
+        for name in Mesh.gradFields:
+            a = getattr(m, name)
+            a += d[name].reshape(a.shape)
+        self.compile()
 
     def _perturb_bc(self, phi, patchID, key, delta):
         from . import BCs

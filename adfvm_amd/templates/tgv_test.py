@@ -1,7 +1,9 @@
 """Taylor-Green vortex case file: kinetic-energy objective, Gaussian source
-perturbation (the shape of templates/cylinder_test.py's perturb).
+perturbation (the shape of templates/cylinder_test.py's perturb) or a
+Gaussian relative perturbation of the mesh arrays.
 
 Environment: ADFVM_CASE_DIR (default cases/tgv<N>), ADFVM_TGV_N (default 12),
+ADFVM_TGV_PARAM (the parameter: source, the default, or mesh),
 ADFVM_ADJ_VISC (adjoint-viscosity type: uniform, abarbanel, turkel; default
 none) with ADFVM_ADJ_SCALING (its scaling, default 0).
 """
@@ -33,11 +35,13 @@ primal = RCF(caseDir,
              )
 
 
-def perturb(fields, mesh, t):
-    # centred where u = U0 (a stagnation point would leave only the
-    # quadratic response, and one-sided differences would not see the slope)
-    mid = np.array([np.pi / 2, np.pi, np.pi])
-    r2 = ((mesh.cellCentres[:mesh.nInternalCells] - mid) ** 2).sum(1, keepdims=True)
+# centred where u = U0 (a stagnation point would leave only the quadratic
+# response, and one-sided differences would not see the slope)
+MID = np.array([np.pi / 2, np.pi, np.pi])
+
+
+def perturb_source(fields, mesh, t):
+    r2 = ((mesh.cellCentres[:mesh.nInternalCells] - MID) ** 2).sum(1, keepdims=True)
     G = 1e-3 * np.exp(-r2 / 0.5)
     rho = G
     rhoU = np.zeros((mesh.nInternalCells, 3))
@@ -46,7 +50,25 @@ def perturb(fields, mesh, t):
     return rho, rhoU, rhoE
 
 
-parameters = 'source'
+def perturb_mesh(fields, mesh, t):
+    """The parameter 'mesh': a relative Gaussian bump of every array of
+    Mesh.gradFields (face arrays by face centre, volumes by cell centre),
+    volumesL/volumesR following the volumes at each face's cells."""
+    nIC, nIF = mesh.nInternalCells, mesh.nInternalFaces
+    bump = lambda x: 1e-6 * np.exp(-((x - MID) ** 2).sum(1) / 0.5)
+    Gf, Gc = bump(mesh.faceCentres[:mesh.nFaces]), bump(mesh.cellCentres[:nIC])
+    d = {name: getattr(mesh, name) * Gf.reshape((-1,) + (1,) * (getattr(mesh, name).ndim - 1))
+         for name in ('areas', 'weights', 'deltas', 'normals', 'deltasUnit', 'linearWeights', 'quadraticWeights')}
+    d['volumes'] = mesh.volumes * Gc[:, None]
+    d['volumesL'] = d['volumes'][mesh.owner]
+    d['volumesR'] = d['volumes'][mesh.neighbour[:nIF]]
+    return [d[name] for name in ('areas', 'volumesL', 'volumesR', 'weights', 'deltas', 'normals', 'deltasUnit',
+                                 'linearWeights', 'quadraticWeights', 'volumes')]
+
+
+# ADFVM_TGV_PARAM: the sensitivity's parameter, 'source' (default) or 'mesh'
+parameters = os.environ.get('ADFVM_TGV_PARAM', 'source')
+perturb = perturb_mesh if parameters == 'mesh' else perturb_source
 adjParams = [float(os.environ.get('ADFVM_ADJ_SCALING', '0')), os.environ.get('ADFVM_ADJ_VISC') or None, None]
 
 nSteps = 20

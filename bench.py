@@ -44,14 +44,14 @@ def kernel_bytes(name, s, nC, nF, nG, nO=None):
     (nO None).  Every boundary face here is cyclic (periodic box)."""
     if name == 'grad_kernel':          # u, V, face (A, n, w) -> 15 gradients
         return nC * (5 + 1 + 15) * s + nF * 5 * s + 54 * nC
-    if name == 'face_flux_kernel':     # u, grad, 17 face rows -> 6 face values
-        return nC * (5 + 15) * s + nF * (17 + 6) * s + 9 * nF
+    if name == 'face_flux_kernel':     # u, grad, 16 face rows (all but the weight) -> 6 face values
+        return nC * (5 + 15) * s + nF * (16 + 6) * s + 9 * nF
     if name == 'stage_kernel':         # face values, V, u_i, u0 (2 of 3 stages) -> u_{i+1}
         return nF * 6 * s + nC * (1 + 5 + 10. / 3 + 5) * s + 30 * nC
     recs = 2 * nF if nO is None else (nF - nG) + nO + 2 * nG
     idx = 8 * nF if nO is None else 4 * (nF - nG) + 8 * nG
-    if name == 'face_vjp_kernel':      # u, grad, ubar, V, 17 face rows -> face adjoint records
-        return nC * (5 + 15 + 5 + 1) * s + nF * 17 * s + recs * 20 * s + 9 * nF + idx
+    if name == 'face_vjp_kernel':      # u, grad, ubar, V, 16 face rows -> face adjoint records
+        return nC * (5 + 15 + 5 + 1) * s + nF * 16 * s + recs * 20 * s + 9 * nF + idx
     if name == 'vjp_gather_kernel':    # face adjoint records -> wbar, gbar; slot flags
         return recs * 20 * s + nC * 20 * s + 6 * nC
     if name == 'grad_vjp_kernel':      # u, wbar, gbar, ubar, acc (2 of 3), V, 6 x 4 slot coefficients -> ubar, acc

@@ -267,6 +267,24 @@ int adfvm_adjoint_step_cotangent(adfvm_ctx* ctx, const void* u_n, double dt, con
                                  double obja, const void* obj_grad, void* source_grad, void* bc_grad,
                                  const void* tape, void* stream);
 
+/* Mesh-geometry gradients (the parameter 'mesh', apps/adjoint.py:107-110:
+ * the step differentiated with respect to Mesh.gradFields, mesh.py:28-32,
+ * each an independent input of the reference's Function).  After
+ * adfvm_set_mesh_grad(ctx, g) with g a device array of
+ * adfvm_mesh_grad_size(ctx) scalars of the context precision, every adjoint
+ * step (adfvm_adjoint_step, adfvm_adjoint_step_cotangent) ADDS
+ * d(obja*J_n + <ua_in, u_{n+1}>)/d(mesh) to g, laid out as the reference's
+ * arrays one after another, row-major:
+ *   areas [nF], volumesL [nF], volumesR [nIF], weights [nF], deltas [nF],
+ *   normals [nF][3], deltasUnit [nF][3], linearWeights [nF][2],
+ *   quadraticWeights [nF][2][3], volumes [nIC]
+ * (the built-in objectives' geometry included; an objective the caller
+ * differentiates itself, adfvm_adjoint_step_cotangent's obj_grad, adds only
+ * its state dependence).  g = NULL switches it off.  Not with cell
+ * segments. */
+int64_t adfvm_mesh_grad_size(const adfvm_ctx* ctx);
+int adfvm_set_mesh_grad(adfvm_ctx* ctx, void* mesh_grad);
+
 /* Adjoint-viscosity stabilisation of the adjoint state after an adjoint step
  * (the viscous map of apps/adjoint.py:127-140 with computeAdjointViscosity
  * and viscositySolver, postpro.py:491-719; the GPU heat-equation solve of

@@ -63,6 +63,9 @@ import numpy as np
 import torch
 
 DT = torch.float64
+# the reference's differentiable mesh arrays, in order (Mesh.gradFields, mesh.py:28-32)
+MESH_GRAD_FIELDS = ('areas', 'volumesL', 'volumesR', 'weights', 'deltas', 'normals', 'deltasUnit',
+                    'linearWeights', 'quadraticWeights', 'volumes')
 CYCLIC_TYPES = ('cyclic', 'slidingPeriodic1D')
 COPY_TYPES = ('zeroGradient', 'empty', 'inletOutlet')          # BCs.py:97-104,135-137
 SYMMETRY_TYPES = ('symmetryPlane', 'slip')                       # BCs.py:120-135
@@ -117,6 +120,11 @@ class Case(object):
         self.cellFaces = torch.as_tensor(m.cellFaces.astype(np.int64))
         self.cellOwner = torch.as_tensor(m.cellOwner.astype(np.int64))
         self.cellNeighbours = torch.as_tensor(m.cellNeighbours.astype(np.int64))
+        # per-face volumes of the divergence (mesh.py:108-109): owner side of
+        # every face, neighbour side of internal faces; separate arrays of the
+        # reference's step (inputs 'volumesL', 'volumesR', mesh.py:28-32)
+        self.t['volumesL'] = self.t['volumes'][self.owner]
+        self.t['volumesR'] = self.t['volumes'][self.neighbour[:self.nIF]]
         # patches: sorted (alphabetical) local patches, mesh.py:100-101
         self.patches = []
         bcs = bcs or {}
@@ -448,20 +456,41 @@ class Case(object):
         return out
 
     def accumulate(self, faceVals, absolute=False):
-        """div / absDiv (adFVM/op.py:12-29): owner += F*(A/V_P), neighbour
-        += -F*(A/V_N) (or +, absolute), summed over the cell's cellFaces
-        slots in order.  faceVals (nF,k) -> (nIC,k)."""
+        """div / absDiv (adFVM/op.py:12-29): owner += F*(A/volumesL),
+        neighbour += -F*(A/volumesR) (or +, absolute), summed over the
+        cell's cellFaces slots in order.  faceVals (nF,k) -> (nIC,k)."""
         A = self.t['areas']
-        V = self.t['volumes']
-        wc = (A[self.cellFaces] / V[:, None])                       # (nIC,6)
-        sign = self.cellOwner.to(faceVals.dtype) * 2 - 1
-        if absolute:
-            sign = torch.ones_like(sign)
+        VL, VR = self.t['volumesL'], self.t['volumesR']
+        own = self.cellOwner.bool()
         acc = 0
         for j in range(6):
             f = self.cellFaces[:, j]
-            acc = acc + faceVals[f] * (sign[:, j] * wc[:, j])[:, None]
+            fr = f.clamp(max=self.nIF - 1)         # neighbour slots are internal faces
+            wP = A[f] / VL[f]
+            wN = A[f] / VR[fr] if self.nIF > 0 else torch.zeros_like(wP)
+            wgt = torch.where(own[:, j], wP, wN if absolute else -wN)
+            acc = acc + faceVals[f] * wgt[:, None]
         return acc
+
+    def mesh_leaves(self):
+        """The step's geometry inputs as autograd leaves, in the order and
+        shapes of the reference's Mesh.gradFields (mesh.py:28-32), the
+        parameter 'mesh' of apps/adjoint.py:107-110: areas (nF,1),
+        volumesL (nF,1), volumesR (nIF,1), weights (nF,1), deltas (nF,1),
+        normals (nF,3), deltasUnit (nF,3), linearWeights (nF,2),
+        quadraticWeights (nF,2,3), volumes (nIC,1).  The case then computes
+        with views of these leaves, each used where the reference uses its
+        array (op.py, interp.py, density.py, BCs.py)."""
+        shapes = {'areas': (self.nF, 1), 'volumesL': (self.nF, 1), 'volumesR': (self.nIF, 1),
+                  'weights': (self.nF, 1), 'deltas': (self.nF, 1), 'normals': (self.nF, 3),
+                  'deltasUnit': (self.nF, 3), 'linearWeights': (self.nF, 2),
+                  'quadraticWeights': (self.nF, 2, 3), 'volumes': (self.nIC, 1)}
+        leaves = []
+        for k in MESH_GRAD_FIELDS:
+            x = self.t[k].detach().clone().reshape(shapes[k]).requires_grad_(True)
+            leaves.append(x)
+            self.t[k] = x.reshape(self.t[k].shape)
+        return leaves
 
     def equation(self, rho, rhoU, rhoE, ghostInit=None, with_fields=False):
         """RCF.equation (density.py:339-414): (drho, drhoU, drhoE, dtc_max)."""

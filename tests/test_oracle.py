@@ -923,3 +923,66 @@ def test_unlimited_second_order_gives_negative_pressure_on_sod():
         f = slice(0, oc.nIF)
         pmin = min(float(oc.second_order(p, gp, s, f).min()) for s in (0, 1))
         assert (pmin < 0) == neg, (limiter, pmin)
+
+
+@pytest.mark.parametrize('name', ['tgv6', 'mixed_upt', 'mixed_totalpt_lf', 'mixed_drag', 'mixed_skew'])
+def test_step_mesh_gradient_autograd_vs_central_differences(name):
+    """Autograd of one step's functional <w, step(u)> + J with respect to the
+    mesh arrays of the reference's parameter 'mesh' (Mesh.gradFields,
+    mesh.py:28-32; apps/adjoint.py:107-110), each an independent input as in
+    the reference's Function: areas, volumesL, volumesR, weights, deltas,
+    normals, deltasUnit, linearWeights, quadraticWeights, volumes.  Each
+    array's gradient along a random direction equals fourth-order central
+    differences of the step with that array perturbed."""
+    from tests import casefactory as cf
+    from tests.test_gpu_primal import CASES
+    import copy
+    case, u = CASES[name]()
+    # a strong constant viscosity (Re ~ 1 on these cells), so that the
+    # viscous flux, the only user of deltas and deltasUnit, moves the state
+    # well above the rounding of the other terms
+    case = copy.copy(case)
+    case.mu0, case.sutherland = 0.5, False
+    rng = np.random.default_rng(21)
+    # a generic state: the exact Taylor-Green field has U_z = 0, which puts
+    # |U.n| and the Roe entropy-fix width on the kink of |.| at 0 on every
+    # z-face (an additive momentum perturbation moves them off it)
+    u = u * (1 + 1e-3 * rng.normal(size=u.shape))
+    u[1:4] += 1e-2 * np.abs(u[1:4]).mean() * rng.normal(size=u[1:4].shape)
+    # each row weighted by the inverse of its scale (rhoE ~ 2.5e5 would
+    # otherwise dominate the functional and its rounding)
+    w = rng.normal(size=u.shape) / np.maximum(np.abs(u).mean(1, keepdims=True), 1.0)
+    gi = [T_(a) for a in cf.ghost_init(case)]
+    obja = 1e-6 if case.objective else 0.0
+    x = T_(u)
+
+    def F(oc):
+        r, rU, rE, dtc, obj = oc.step(x[0:1].T, x[1:4].T, x[4:5].T, 1e-4, ghostInit=gi)
+        return (torch.cat([r.T, rU.T, rE.T]) * T_(w)).sum() + obja * obj
+
+    oc = case.oracle()
+    leaves = oc.mesh_leaves()
+    grads = torch.autograd.grad(F(oc), leaves, allow_unused=True)
+    base = [l.detach().clone() for l in leaves]
+    for k, (fname, g) in enumerate(zip(fvm_oracle.MESH_GRAD_FIELDS, grads)):
+        v0 = base[k]
+        dv = T_(rng.normal(size=tuple(v0.shape))) * (v0.abs().mean() + 1e-12)
+        ad = float((g * dv).sum()) if g is not None else 0.0
+
+        def at(s, eps):
+            o2 = case.oracle()
+            vals = [b.clone() for b in base]
+            vals[k] = vals[k] + s * eps * dv
+            for kk, (f2, val) in enumerate(zip(fvm_oracle.MESH_GRAD_FIELDS, vals)):
+                o2.t[f2] = val.reshape(o2.t[f2].shape)
+            return float(F(o2))
+
+        # three steps: the larger ones where rounding dominates, the
+        # smallest where a kink (the LF wave-speed maximum, |U.n|) lies
+        # within a larger step
+        errs = []
+        for eps in (1e-4, 1e-5, 1e-6):
+            fd = (8 * (at(1, eps) - at(-1, eps)) - (at(2, eps) - at(-2, eps))) / (12 * eps)
+            errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+        print(name, fname, 'ad %.6e rel %.1e %.1e %.1e' % (ad, errs[0], errs[1], errs[2]))
+        assert min(errs) <= 1e-6, (fname, ad, errs)
