@@ -105,6 +105,8 @@ SIGNATURES = {
     'adfvm_interp_second_order': (C.c_int, [vp, vp, vp, C.c_int32, C.c_int32, vp, vp]),
     'adfvm_dot': (C.c_int, [vp, vp, vp, C.c_int64, vp, vp]),
     'adfvm_adjoint_energy': (C.c_int, [vp, vp, vp, vp, vp]),
+    'adfvm_dot_noalloc': (C.c_int, [C.c_int32, vp, vp, C.c_int64, vp, vp]),
+    'adfvm_noalloc_last_error': (C.c_char_p, []),
     'adfvm_mesh_grad_size': (C.c_int64, [vp]),
     'adfvm_set_mesh_grad': (C.c_int, [vp, vp]),
     'adfvm_profile': (C.c_int, [vp, C.c_int32, vp]),

@@ -41,6 +41,31 @@ def _stream(stream):
     return C.c_void_p(stream)
 
 
+def dot_noalloc(a, b, out, stream=None):
+    """sum_j a[j] b[j] in FP64 into out[0] through adfvm_dot_noalloc: no
+    context, and neither the library nor this wrapper allocates device memory
+    (out is the caller's).  a, b: contiguous HIP-device float32 or float64
+    tensors of one dtype and element count (at most 2^27); out: a contiguous
+    float64 HIP-device tensor.  Calls on one GPU share the library's partial
+    buffer, so calls overlapping in time on one GPU race."""
+    import torch
+    for name, t in (('a', a), ('b', b)):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous() or \
+                t.dtype not in (torch.float32, torch.float64):
+            raise ValueError('%s: expected a contiguous float32/float64 HIP-device tensor' % name)
+    if a.dtype != b.dtype or a.numel() != b.numel():
+        raise ValueError('a, b: dtypes %s/%s, %d/%d elements' % (a.dtype, b.dtype, a.numel(), b.numel()))
+    if not isinstance(out, torch.Tensor) or out.dtype != torch.float64 or not out.is_cuda or \
+            not out.is_contiguous() or out.numel() < 1:
+        raise ValueError('out: needs a contiguous float64 HIP-device tensor')
+    lib = _lib.load()
+    rc = lib.adfvm_dot_noalloc(4 if a.dtype == torch.float32 else 8, _dptr(a), _dptr(b), a.numel(), _dptr(out),
+                               _stream(stream))
+    if rc != 0:
+        raise _lib.LibraryError('libadfvm error %d: %s' % (rc, lib.adfvm_noalloc_last_error().decode()))
+    return out
+
+
 class Context(object):
     """Owns an adfvm_ctx.
 

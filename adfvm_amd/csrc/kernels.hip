@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1469,6 +1470,73 @@ extern "C" int adfvm_dot(adfvm_ctx* c, const void* a, const void* b, int64_t n, 
     else dot_kernel<double><<<g, BLOCK, 0, s>>>((const double*)a, (const double*)b, n, c->partial);
     finalize_kernel<0><<<1, BLOCK, 0, s>>>(c->partial, g, out);
     HIPCHK(c, hipGetLastError());
+    return ADFVM_OK;
+}
+
+// ---- the allocation-free dot product ------------------------------------
+// Block partials of adfvm_dot_noalloc live in this file-scope array, one
+// copy per GPU, held until the process ends.  In host code its name is not a
+// device address: the address each device holds it at is looked up once per
+// device (hipGetSymbolAddress with that device current) and only that
+// pointer is passed to kernels.
+constexpr long kScratchDoubles = 1L << 19;        // 4 MiB: n <= 2^19 * 256 scalars
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices];
+
+// The current device's copy of scratch (nullptr and a message on failure).
+double* scratch_on_current_device(std::string& err) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        err = "no current HIP device (or device index >= 64)";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+thread_local std::string g_noalloc_err;
+}  // namespace
+
+extern "C" const char* adfvm_noalloc_last_error(void) { return g_noalloc_err.c_str(); }
+
+extern "C" int adfvm_dot_noalloc(int32_t precision, const void* a, const void* b, int64_t n, double* out,
+                                 void* stream) {
+    g_noalloc_err.clear();
+    if ((n > 0 && (!a || !b)) || !out || n < 0 || (precision != 4 && precision != 8)) {
+        g_noalloc_err = "adfvm_dot_noalloc: null pointer, negative n or precision not 4/8";
+        return ADFVM_ERR_INVALID;
+    }
+    const long g = std::max<long>(grid_for(n), 1);
+    if (g > kScratchDoubles) {
+        g_noalloc_err = "adfvm_dot_noalloc: n = " + std::to_string((long long)n) + " exceeds the maximum " +
+                        std::to_string((long long)(kScratchDoubles * BLOCK)) + " scalars";
+        return ADFVM_ERR_INVALID;
+    }
+    double* part = scratch_on_current_device(g_noalloc_err);
+    if (!part) return ADFVM_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    // every one of the g partials is written by dot_kernel before
+    // finalize_kernel reads it, so nothing needs clearing
+    if (precision == 4) dot_kernel<float><<<(int)g, BLOCK, 0, s>>>((const float*)a, (const float*)b, n, part);
+    else dot_kernel<double><<<(int)g, BLOCK, 0, s>>>((const double*)a, (const double*)b, n, part);
+    finalize_kernel<0><<<1, BLOCK, 0, s>>>(part, (int)g, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_noalloc_err = std::string("adfvm_dot_noalloc: ") + hipGetErrorString(e);
+        return ADFVM_ERR_HIP;
+    }
     return ADFVM_OK;
 }
 

@@ -377,6 +377,21 @@ int adfvm_interp_second_order(adfvm_ctx* ctx, const void* phi, const void* grad,
 /* Reductions with FP64 accumulation in a fixed order.
  * sensitivity = sum_j a[j]*b[j] (cmesh.cpp:309-340); n scalars, device. */
 int adfvm_dot(adfvm_ctx* ctx, const void* a, const void* b, int64_t n, double* out, void* stream);
+/* adfvm_dot without a context and without allocating: the same kernels and
+ * the same fixed summation order as adfvm_dot (bitwise equal to it for n up
+ * to that context's partial count x 256), with the block partials in a
+ * file-scope device array of 4 MiB per GPU.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, allocation-free serving loops).  precision is 4
+ * or 8 (a, b float or double; NULL allowed when n = 0, giving 0);
+ * n <= 2^27 (= 134217728) scalars, larger n is
+ * rejected with ADFVM_ERR_INVALID.  The first call on a device looks up that
+ * device's copy of the array (make it before a capture).  Calls on one GPU
+ * share that array: calls that overlap in time on one GPU (different streams
+ * or threads) race and must be serialised by the caller.  Errors:
+ * adfvm_noalloc_last_error() (per host thread). */
+int adfvm_dot_noalloc(int32_t precision, const void* a, const void* b, int64_t n, double* out, void* stream);
+const char* adfvm_noalloc_last_error(void);
 /* adjoint energy sum_c w^T A0 w V_c with w = ua/V (cmesh.cpp:342-425). */
 int adfvm_adjoint_energy(adfvm_ctx* ctx, const void* ua, const void* u, double* out, void* stream);
 
