@@ -256,6 +256,7 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
     int c = xcd_block() * BS + threadIdx.x;
     if (c >= m.nIC) return;
     const long nIC = m.nIC;
+    const T V = m.vol[c];       // loaded first: read after the slot loop it was a last, unhidden wait
     T wc[5];
     load_prim(ph, u, nIC, ghost, m.nBF, c, wc);
     T g[15];
@@ -283,7 +284,6 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
             g[3 * i + 2] += phiF * SN[2];
         }
     }
-    T V = m.vol[c];
     T out[GREC];
     for (int i = 0; i < 15; i++) out[i] = g[i] / V;
     out[15] = T(0);
