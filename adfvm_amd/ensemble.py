@@ -7,8 +7,9 @@ instances share the GPU instead of each under-filling it, and each instance's
 arithmetic is exactly what it would be alone (own cell order, own slot
 tables, own geometry).  The context evaluates the objective per instance
 (`Context.set_segments`).  The reverse sweep is the uniform checkpoint
-schedule with HIP-graph segments (`graphs.graph_sweep`): a whole segment is
-one graph launch.  Instances are independent, so the
+schedule in HIP graphs (`graphs.GraphSweep`): by default the whole sweep
+is one graph launch (or one graph per checkpoint segment).  Instances are
+independent, so the
 adjoint of the summed objective gives every instance its own gradient.
 
     ens = TaylorGreenEnsemble(n_instances=64, sizes=(32, 48, 64, 96), seed=0)
@@ -68,13 +69,16 @@ class TaylorGreenEnsemble(object):
     def n_cells(self):
         return self.mesh.nInternalCells
 
-    def run(self, n_steps=200, interval=20, graphs=True, tapes=True):
+    def run(self, n_steps=200, interval=20, graphs=True, tapes=True, whole=None):
         """Primal + adjoint of J_k = sum over steps of instance k's kinetic
         energy (on each step's stage-1 state).  Returns (J (nSeg,) FP64
         host array, ua: d(sum_k J_k)/du0 on the device, i.e. each instance's
-        own gradient)."""
+        own gradient).  With graphs, the whole sweep is one captured HIP
+        graph, a single launch per run (whole=False: one graph per
+        checkpoint segment, replayed per segment)."""
         from .graphs import GraphSweep
-        key = (interval, graphs, tapes)
+        whole = graphs if whole is None else whole
+        key = (interval, graphs, tapes, whole)
         if key not in self._sweeps:                 # captured once, replayed by later runs
-            self._sweeps[key] = GraphSweep(self.ctx, self.dt, interval, 1.0, graphs, tapes)
+            self._sweeps[key] = GraphSweep(self.ctx, self.dt, interval, 1.0, graphs, tapes, whole=whole)
         return self._sweeps[key].run(self.u0, n_steps)

@@ -9,6 +9,10 @@ one graph launch instead of a few hundred kernel launches -- what matters on
 meshes small enough for launch overhead to show (the cylinder O-grid, small
 ensemble members).  The reverse sweep is the uniform checkpoint schedule of
 apps/adjoint.py:205-217 (checkpoints every writeInterval steps).
+
+With whole=True the entire sweep for a fixed number of steps -- the forward
+run writing its checkpoints into a preallocated buffer, then every reverse
+segment -- is one captured graph, replayed as a single launch per sweep.
 """
 import numpy as np
 
@@ -21,11 +25,14 @@ class GraphSweep(object):
     gradient d(obja * J)/d(BC values) is accumulated into self.bc_grad
     ((5, nBoundaryFaces), Context.bc_param locates a parameter in it)."""
 
-    def __init__(self, ctx, dt, interval, obja=1.0, graphs=True, tapes=True, source=None, bc_grad=False):
+    def __init__(self, ctx, dt, interval, obja=1.0, graphs=True, tapes=True, source=None, bc_grad=False,
+                 whole=False):
         torch = ctx.torch
         self.torch, self.ctx = torch, ctx
         self.dt, self.interval, self.obja, self.source = float(dt), int(interval), float(obja), source
         self.graphs = graphs
+        self.whole = whole                  # the whole sweep as one graph (needs graphs)
+        self._whole = None                  # (n_steps, graph, checkpoint buffer, input buffer)
         nseg = getattr(ctx, 'nSeg', 1)
         shape = (5, ctx.nIC)
         self.u = torch.empty(shape, dtype=ctx.tdtype, device='cuda')
@@ -70,6 +77,61 @@ class GraphSweep(object):
             self._reverse_segment()
         self._fwd, self._rev = g_fwd, g_rev
 
+    def _sweep(self, u0, n_steps, run_fwd, run_rev, ckpt=None):
+        """The sweep's work, in the order a single run issues it: zeroed
+        accumulators, the forward segments keeping each start state, then
+        the reverse segments last first.  ckpt (n_seg, 5, nIC): checkpoint
+        buffer (else fresh tensors)."""
+        u = self.u
+        u.copy_(u0)
+        self.J.zero_()
+        self.ua.zero_()
+        if self.bc_grad is not None:
+            self.bc_grad.zero_()
+        nseg = n_steps // self.interval
+        checkpoints = []
+        for s in range(nseg):
+            if ckpt is not None:
+                ckpt[s].copy_(u)
+                checkpoints.append(ckpt[s])
+            else:
+                checkpoints.append(u.clone())
+            run_fwd()
+        if self.keep_final:
+            if ckpt is not None:
+                self.u_final.copy_(u)
+            else:
+                self.u_final = u.clone()
+        while checkpoints:
+            u.copy_(checkpoints.pop())
+            run_rev()
+
+    def _run_whole(self, u0, n_steps):
+        torch = self.torch
+        if self._whole is None or self._whole[0] != n_steps:
+            self._whole = None
+            nseg = n_steps // self.interval
+            ckpt = torch.empty((nseg,) + tuple(self.u.shape), dtype=self.u.dtype, device='cuda')
+            uin = torch.empty_like(self.u)
+            if self.keep_final:
+                self.u_final = torch.empty_like(self.u)
+            uin.copy_(u0)
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):        # warm-up outside capture
+                self._forward_segment()
+                self._reverse_segment()
+            torch.cuda.current_stream().wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._sweep(uin, n_steps, self._forward_segment, self._reverse_segment, ckpt)
+            self._whole = (n_steps, g, ckpt, uin)
+        _, g, ckpt, uin = self._whole
+        uin.copy_(u0)
+        g.replay()
+        torch.cuda.synchronize()
+        return np.asarray(self.J.cpu().numpy()), self.ua
+
     def run(self, u0, n_steps):
         """Forward n_steps steps from the device state u0 (unchanged), then
         the reverse sweep.  Returns (J, ua): J the FP64 sum over steps of the
@@ -78,31 +140,19 @@ class GraphSweep(object):
         run)."""
         if n_steps % self.interval:
             raise ValueError('n_steps must be a multiple of interval')
+        if self.whole:
+            if not self.graphs:
+                raise ValueError('whole=True captures the sweep as one graph: it needs graphs=True')
+            return self._run_whole(u0, n_steps)
         if self.graphs and self._fwd is None:
             self._capture(u0)
         run_fwd = self._fwd.replay if self.graphs else self._forward_segment
         run_rev = self._rev.replay if self.graphs else self._reverse_segment
-        u = self.u
-        # forward sweep: objective, and the state at every segment start
-        u.copy_(u0)
-        self.J.zero_()
-        self.ua.zero_()
-        if self.bc_grad is not None:
-            self.bc_grad.zero_()
-        checkpoints = []
-        for s in range(n_steps // self.interval):
-            checkpoints.append(u.clone())
-            run_fwd()
-        if self.keep_final:
-            self.u_final = u.clone()
-        # reverse sweep, last segment first
-        while checkpoints:
-            u.copy_(checkpoints.pop())
-            run_rev()
+        self._sweep(u0, n_steps, run_fwd, run_rev)
         self.torch.cuda.synchronize()
         return np.asarray(self.J.cpu().numpy()), self.ua
 
 
-def graph_sweep(ctx, u0, dt, n_steps, interval, obja=1.0, graphs=True, tapes=True, source=None):
+def graph_sweep(ctx, u0, dt, n_steps, interval, obja=1.0, graphs=True, tapes=True, source=None, whole=False):
     """One sweep (GraphSweep(...).run(u0, n_steps))."""
-    return GraphSweep(ctx, dt, interval, obja, graphs, tapes, source).run(u0, n_steps)
+    return GraphSweep(ctx, dt, interval, obja, graphs, tapes, source, whole=whole).run(u0, n_steps)

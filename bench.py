@@ -216,8 +216,8 @@ def cpu_baseline(mesh, mu0, u0, dt, steps=8):
 def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
     """Batched ensemble: n_instances Taylor-Green boxes, sizes drawn by seed
     from {32, 48, 64, 96}^3, +-1% seeded velocity perturbation, advanced
-    primal + adjoint as one concatenated mesh, one HIP graph per checkpoint
-    segment, FP32 state, FP64 per-instance objectives."""
+    primal + adjoint as one concatenated mesh, the whole sweep one captured
+    HIP graph (one launch), FP32 state, FP64 per-instance objectives."""
     from adfvm_amd.ensemble import TaylorGreenEnsemble
     t0 = time.time()
     ens = TaylorGreenEnsemble(n_instances=n_instances, seed=seed)
@@ -225,7 +225,7 @@ def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
                                                                            time.time() - t0),
           file=sys.stderr, flush=True)
     tuned = ens.ctx.autotune(ens.u0, reps=2)
-    ens.run(n_steps=interval, interval=interval)                  # warm-up (captures, runs one segment)
+    ens.run(n_steps=n_steps, interval=interval)                   # warm-up: captures the sweep's graph, runs it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J, ua = ens.run(n_steps=n_steps, interval=interval)
@@ -237,8 +237,8 @@ def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
             'ms_per_step': t / n_steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
             'dtype': 'f32', 'data': 'synthetic (Taylor-Green initial fields, seeded +-1% velocity perturbation)',
             'config': {'workload': 'batched ensemble: %d Taylor-Green instances, sizes %s^3 drawn by seed %d, '
-                                   'one concatenated mesh, checkpoints every %d steps, each segment one HIP '
-                                   'graph' % (n_instances, sorted(set(ens.sizes)), seed, interval),
+                                   'one concatenated mesh, checkpoints every %d steps, the whole sweep one '
+                                   'HIP graph launch' % (n_instances, sorted(set(ens.sizes)), seed, interval),
                        'n_cells': ens.n_cells, 'instances': n_instances,
                        'size_counts': {str(k): ens.sizes.count(k) for k in sorted(set(ens.sizes))},
                        'dt': ens.dt},

@@ -45,16 +45,22 @@ def test_concatenated_mesh_reproduces_instances():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('graphs', [False, True])
-def test_ensemble_matches_instances_bitwise(graphs):
+@pytest.mark.parametrize('mode', ['eager', 'segment_graphs', 'whole_graph'])
+def test_ensemble_matches_instances_bitwise(mode):
     torch = pytest.importorskip('torch')
     from adfvm_amd.ensemble import TaylorGreenEnsemble
     from adfvm_amd.context import Context
     from adfvm_amd import adjoint, _lib, config
     ens = TaylorGreenEnsemble(n_instances=5, sizes=(6, 8, 10), seed=3, precision=np.float32)
     print('ensemble sizes', ens.sizes, 'cells', ens.n_cells, flush=True)
-    n_steps, interval = 9, 3                     # three replays of each graph
-    J, ua = ens.run(n_steps=n_steps, interval=interval, graphs=graphs)
+    n_steps, interval = 9, 3                     # three replays of each segment graph
+    graphs, whole = mode != 'eager', mode == 'whole_graph'
+    J, ua = ens.run(n_steps=n_steps, interval=interval, graphs=graphs, whole=whole)
+    if whole:
+        # a second replay of the captured sweep gives the same bits
+        ua = ua.clone()
+        J2, ua2 = ens.run(n_steps=n_steps, interval=interval, graphs=graphs, whole=whole)
+        assert np.array_equal(J, J2) and torch.equal(ua, ua2)
     assert J.shape == (5,) and np.all(J > 0)
     ua = ua.cpu().numpy()
     for k, n in enumerate(ens.sizes):
@@ -110,7 +116,7 @@ def test_ensemble_full_size_workload():
     instances, sizes drawn by seed 0 from {32, 48, 64, 96}^3, each with a
     seeded +-1% velocity perturbation, 200 steps primal + adjoint of each
     instance's summed kinetic energy in FP32 (FP64 objectives), checkpoints
-    every 20 steps, every segment one HIP graph.  Checks:
+    every 20 steps, the whole sweep one captured HIP graph.  Checks:
     (1) every instance conserves sum V rho, sum V rhoU, sum V rhoE over the
         200 steps to FP32 round-off (relative <= 1e-6);
     (2) the first 32^3 instance's state after 200 steps and its objective
@@ -132,7 +138,7 @@ def test_ensemble_full_size_workload():
     ens = TaylorGreenEnsemble(n_instances=64, seed=0, precision=np.float32)
     sizes = ens.sizes
     print('ensemble: %d cells, sizes %s' % (ens.n_cells, {s: sizes.count(s) for s in sorted(set(sizes))}), flush=True)
-    sweep = GraphSweep(ens.ctx, ens.dt, interval, 1.0, True, True)
+    sweep = GraphSweep(ens.ctx, ens.dt, interval, 1.0, True, True, whole=True)
     sweep.keep_final = True
     J, ua = sweep.run(ens.u0, n_steps)
     assert np.isfinite(J).all() and bool(torch.isfinite(ua).all())
