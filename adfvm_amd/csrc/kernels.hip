@@ -141,14 +141,15 @@ __device__ __forceinline__ void load_prim(const Phys<T>& ph, const T* __restrict
     for (int i = 0; i < 5; i++) w[i] = internal ? p[i] : q[i];
 }
 
-// Internal work buffers are arrays of padded records so that a gathered
-// record is a few 16-byte vector loads instead of one cache line per field:
-//   gradients        [nIC][GREC]   15 used
-//   face fluxes      [nF][FREC]    6 used (F*A, lam*A)
+// Internal work buffers are arrays of records so that a gathered record is a
+// few vector loads instead of one cache line per field:
+//   gradients        [nIC][GREC]   15 used (padded to 16-byte multiples)
+//   face fluxes      [nF][FREC]    F*A (5), lam*A, unpadded: 24 B in FP32 (48 B in
+//                                  FP64), accessed as pairs of scalars
 //   face adjoints    [nPlanes][nIC][AREC] one plane per cell slot, then cyclic
 //                                  extras: gradient adjoint (15), primitive adjoint (5)
 //   cell adjoints    [nIC][AREC]   same layout as one side of a face record
-constexpr int GREC = 16, FREC = 8, AREC = 20;
+constexpr int GREC = 16, FREC = 6, AREC = 20;
 
 // N contiguous values from / to a 16-byte aligned address (N a multiple of
 // 16/sizeof(T)), as 16-byte memory operations.  The copies go through
@@ -166,6 +167,21 @@ template <int N, typename T> __device__ __forceinline__ void vstore(T* __restric
     T* a = static_cast<T*>(__builtin_assume_aligned(p, 16));
 #pragma unroll
     for (int k = 0; k < (int)(N * sizeof(T) / 16); k++) __builtin_memcpy(a + k * (16 / sizeof(T)), in + k * (16 / sizeof(T)), 16);
+}
+
+// Records of an even number of scalars aligned to a pair of them (the face
+// flux records): pairs of scalars per access.
+template <int N, typename T> __device__ __forceinline__ void pload(const T* __restrict__ p, T* out) {
+    static_assert(N % 2 == 0, "pairs");
+    const T* a = static_cast<const T*>(__builtin_assume_aligned(p, 2 * sizeof(T)));
+#pragma unroll
+    for (int k = 0; k < N / 2; k++) __builtin_memcpy(out + 2 * k, a + 2 * k, 2 * sizeof(T));
+}
+template <int N, typename T> __device__ __forceinline__ void pstore(T* __restrict__ p, const T* in) {
+    static_assert(N % 2 == 0, "pairs");
+    T* a = static_cast<T*>(__builtin_assume_aligned(p, 2 * sizeof(T)));
+#pragma unroll
+    for (int k = 0; k < N / 2; k++) __builtin_memcpy(a + 2 * k, in + 2 * k, 2 * sizeof(T));
 }
 
 // the 15 gradient entries of cell c (g has room for GREC values)
@@ -377,8 +393,7 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
     T rec[FREC];
     for (int i = 0; i < 5; i++) rec[i] = F[i] * fg.A;
     rec[5] = lam * fg.A;
-    rec[6] = rec[7] = T(0);
-    vstore<FREC>(fbuf + (long)FREC * f, rec);
+    pstore<FREC>(fbuf + (long)FREC * f, rec);
     (void)nF;
 }
 
@@ -415,7 +430,7 @@ __global__ void __launch_bounds__(BS) stage_kernel(DMesh<T> m, const T* __restri
             const int f = ld<ST>(m.sFace + j * nIC + c);
             const bool own = ld<ST>(m.sFlags + j * nIC + c) & 1;
             T rec[FREC];
-            vload<FREC>(fbuf + (long)FREC * f, rec);
+            pload<FREC>(fbuf + (long)FREC * f, rec);
             for (int i = 0; i < 5; i++) L[i] += own ? rec[i] : -rec[i];
             lamSum += rec[5];
         }

@@ -111,12 +111,13 @@ def test_primal_map_uses_positional_source_and_bc_values():
         s.map(*positional_inputs(s, u, dt)[:-1])
 
 
-@pytest.mark.parametrize('param', ['source', ('BCs', 'U', 'inlet', 'value'), ('BCs', 'p', 'outlet', 'p0')])
+@pytest.mark.parametrize('param', ['source', 'mesh', ('BCs', 'U', 'inlet', 'value'), ('BCs', 'p', 'outlet', 'p0')])
 def test_adjoint_map_positional_contract(param):
     """Adjoint.map (primal_grad) with the reference's positional list:
-    adjoint fields and the parameter gradient equal Context.adjoint_step's
-    bitwise; the gradient accumulates over calls until zero_static; a
-    non-zero dtc seed raises."""
+    adjoint fields and the parameter gradient (the three source arrays, the
+    ten Mesh.gradFields arrays, or the BC value array) equal
+    Context.adjoint_step's bitwise; the gradient accumulates over calls
+    until zero_static; a non-zero dtc seed raises."""
     from adfvm_amd.adjoint import Adjoint
     case, u, _ = cf.bc_param_cases()['upt_drag']()
     s = cf.rcf(case, u, np.float64)
@@ -133,14 +134,22 @@ def test_adjoint_map_positional_contract(param):
     ud = torch.as_tensor(u, device='cuda').contiguous()
     uad = torch.as_tensor(ua, device='cuda').contiguous()
     sg, bg = ctx.state(0.), ctx.bc_state()
+    mg = ctx.mesh_grad_state() if param == 'mesh' else None
     sd = torch.as_tensor(src, device='cuda').contiguous() if src is not None else None
     for _ in range(2):
         ua_i = uad.clone()
         ctx.adjoint_step(ud, dt, ua_i, obja=1.0, source=sd, source_grad=sg, bc_grad=bg)
+    if mg is not None:
+        ctx.set_mesh_grad(None)
     got = np.concatenate([out2[0].T, out2[1].T, out2[2].T])
     assert np.array_equal(got, ua_i.cpu().numpy())
     assert out1[3] is None
-    if param == 'source':
+    if param == 'mesh':
+        assert len(out2) == 3 + 10 and all(a is None for a in out1[3:])
+        for a, b in zip(out2[3:], ctx.mesh_grad_arrays(mg)):
+            assert a.shape == tuple(b.shape) and np.array_equal(a, b.cpu().numpy())
+        assert max(np.abs(a).max() for a in out2[3:]) > 0
+    elif param == 'source':
         g = sg.cpu().numpy()
         assert np.array_equal(out2[3], g[0:1].T) and np.array_equal(out2[4], g[1:4].T) and \
             np.array_equal(out2[5], g[4:5].T)
