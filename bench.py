@@ -374,7 +374,7 @@ def main():
                     help='only run a device-to-device copy of MIB MiB five times (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)' % args.gpus,
+        print('bench.py runs on one GPU only: the library has no multi-GPU path (--gpus %d requested)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, 'cpu_step.cpp')
 OUT_DIR = os.path.join(HERE, '_cpu')
 LIB = os.path.join(OUT_DIR, 'libadfvm_cpu.so')
-# x86-64-v3 (AVX2, FMA): the host of the GPU box runs the library built here
+# x86-64-v3 (AVX2, FMA), not -march=native: the library is built on one host
+# and timed on another (the MI355X machine's host CPU)
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', '-fopenmp', '-march=x86-64-v3', '-fno-math-errno',
          '-I', os.path.join(ROOT, 'adfvm_amd', 'csrc')]
 
