@@ -21,6 +21,7 @@ RIEMANN_ROE, RIEMANN_LF = 0, 1
 MU_CONSTANT, MU_SUTHERLAND = 0, 1
 RK_SSPRK3, RK_EULER = 0, 1
 OBJ_NONE, OBJ_KINETIC_ENERGY, OBJ_DRAG = 0, 1, 2
+OBJSTATE_STAGE1, OBJSTATE_START = 0, 1
 
 i32p = C.POINTER(C.c_int32)
 f64p = C.POINTER(C.c_double)
@@ -83,6 +84,8 @@ SIGNATURES = {
     'adfvm_ctx_precision': (C.c_int, [vp]),
     'adfvm_set_segments': (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64)]),
     'adfvm_segments': (C.c_int, [vp]),
+    'adfvm_set_objective_state': (C.c_int, [vp, C.c_int32]),
+    'adfvm_objective_state': (C.c_int, [vp]),
     'adfvm_autotune': (C.c_int, [vp, vp, C.c_int32, vp]),
     'adfvm_kernel_config': (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_double)]),

@@ -223,6 +223,22 @@ class Context(object):
         self._chk(self.lib.adfvm_set_segments(self.handle, len(a) - 1, _lib.ptr(a, C.c_int64)))
         self.nSeg = len(a) - 1
 
+    OBJECTIVE_STATES = {'stage1': _lib.OBJSTATE_STAGE1, 'start': _lib.OBJSTATE_START}
+
+    @property
+    def objective_state(self):
+        """The state each step's objective is evaluated on: 'stage1' (the
+        reference's hook, the first intermediate RK stage, density.py:412-413;
+        the default) or 'start' (the step's start state u_n)."""
+        code = self.lib.adfvm_objective_state(self.handle)
+        return {v: k for k, v in self.OBJECTIVE_STATES.items()}[code]
+
+    @objective_state.setter
+    def objective_state(self, which):
+        if which not in self.OBJECTIVE_STATES:
+            raise ValueError('objective state %r (stage1, start)' % (which,))
+        self._chk(self.lib.adfvm_set_objective_state(self.handle, self.OBJECTIVE_STATES[which]))
+
     def _check_out64(self, t, n, name):
         if t is None:
             return
@@ -317,8 +333,8 @@ class Context(object):
         """The state the objective is evaluated on in the step from u_n
         (density.py:412-413): the first intermediate stage, which the step's
         tape holds first (a view into it), or u_n itself for a one-stage
-        integrator."""
-        if self.n_stages == 1:
+        integrator or with objective_state 'start'."""
+        if self.n_stages == 1 or self.objective_state == 'start':
             return u_n
         if tape is None:
             raise ValueError('the objective stage of a multi-stage step is read from its tape')

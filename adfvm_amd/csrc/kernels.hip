@@ -694,6 +694,7 @@ struct adfvm_ctx {
     int nPatches = 0;
     bool hasGhostKernel = false;
     int objective = ADFVM_OBJ_NONE, objectivePatch = -1;
+    int objState = ADFVM_OBJSTATE_STAGE1;   // the state the objective is evaluated on
     int integrator = ADFVM_RK_SSPRK3;
     Phys<double> phd;
     std::string err;
@@ -1057,6 +1058,15 @@ void stages(int integrator, std::vector<Stage>& st) {
 // A step tape holds the forward step's intermediate stage states (all
 // stages but the first, whose state is the step's input) and every stage's
 // gradient records: [(ns-1) x 5 x nIC][ns x GREC x nIC] scalars.
+// The stage whose state the step's objective is evaluated on: the first
+// intermediate stage (density.py:412-413; the state a tape holds first), or
+// the step's start state u_n when the caller asked for it
+// (ADFVM_OBJSTATE_START; no counterpart in the reference).  dtc is taken on
+// the first intermediate stage either way.
+int objective_stage(const adfvm_ctx* c, int ns) {
+    return (ns > 1 && c->objState == ADFVM_OBJSTATE_STAGE1) ? 1 : 0;
+}
+
 long tape_scalars(const adfvm_ctx* c, int ns) { return ((long)(ns - 1) * 5 + (long)ns * GREC) * c->nIC; }
 
 template <typename T> void tape_slots(const adfvm_ctx* c, int ns, T* tape, T** states, T** grads) {
@@ -1075,7 +1085,8 @@ int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, do
     T* grads[3] = {P<T>(c->grad0), P<T>(c->grad1), P<T>(c->grad2)};
     if (tape) tape_slots(c, ns, tape, bufs, grads);
     const T* ui = u;
-    const int objStage = ns > 1 ? 1 : 0;    // density.py:412-413
+    const int objStage = objective_stage(c, ns);
+    const int dtcStage = ns > 1 ? 1 : 0;    // density.py:412-413
     for (int i = 0; i < ns; i++) {
         // the last stage writes u in place (each cell reads only its own u0
         // entry); a single-stage scheme reads u's neighbours, so it goes
@@ -1083,7 +1094,7 @@ int primal_step_t(adfvm_ctx* c, T* u, double dt, const T* S, double* dtc_out, do
         T* out = (i == ns - 1 && ns > 1) ? u : bufs[i];
         if (i == objStage && obj_out) launch_objective<T>(c, ui, obj_out, s);
         launch_grad<T>(c, ui, grads[i], s);
-        const bool wantDtc = (i == objStage) && dtc_out;
+        const bool wantDtc = (i == dtcStage) && dtc_out;
         const T* u0 = (i > 0 && st[i].a0 != 0.0) ? (const T*)u : nullptr;
         int g = grid_bs(c->nIC, c->bs[K_STAGE]);
         launch_flux_and_stage<T, 0>(c, ui, grads[i], u0, st[i], dt, S, out, nullptr, wantDtc ? c->partial : nullptr, s);
@@ -1388,6 +1399,16 @@ extern "C" int adfvm_set_segments(adfvm_ctx* c, int32_t nSeg, const int64_t* cel
 }
 
 extern "C" int adfvm_segments(const adfvm_ctx* c) { return c ? c->nSeg : 0; }
+
+extern "C" int adfvm_set_objective_state(adfvm_ctx* c, int32_t which) {
+    if (!c) return ADFVM_ERR_INVALID;
+    if (which != ADFVM_OBJSTATE_STAGE1 && which != ADFVM_OBJSTATE_START)
+        return set_err(c, ADFVM_ERR_INVALID, "objective state must be ADFVM_OBJSTATE_STAGE1 or ADFVM_OBJSTATE_START");
+    c->objState = which;
+    return ADFVM_OK;
+}
+
+extern "C" int adfvm_objective_state(const adfvm_ctx* c) { return c ? c->objState : -1; }
 
 #define DISPATCH(c, CALL_F, CALL_D) ((c)->prec == 4 ? (CALL_F) : (CALL_D))
 

@@ -252,7 +252,9 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
     reference's objective hook on the stage-1 state), uniform checkpoints
     every `interval` steps, each checkpoint segment one HIP graph; FP32
     kernels with FP64 objective accumulation, and the same run in FP64 for
-    the error and speed ratio."""
+    the error and speed ratio.  The same sweeps with the objective on each
+    step's start state u_n are reported beside them (objective_state
+    'start')."""
     from adfvm_amd.graphs import GraphSweep
     from tests import casefactory as cf
     t0 = time.time()
@@ -264,23 +266,28 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
     print('[bench] TGV %d^3 adjoint, %d cells, built in %.1fs' % (n, nC, time.time() - t0), file=sys.stderr,
           flush=True)
     res, tuned = {}, None
-    for prec in (np.float32, np.float64):
-        ctx = case.context(prec)
-        u0 = torch.as_tensor(u0h.astype(prec), device='cuda').contiguous()
-        if prec == np.float32:
-            tuned = ctx.autotune(u0, reps=3)
-        sweep = GraphSweep(ctx, dt, interval, obja=1.0)
-        sweep.run(u0, interval)                    # capture + one segment
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        J, ua = sweep.run(u0, n_steps)
-        t = time.perf_counter() - t1
-        res[prec] = (float(J[0]), ua.double().cpu().numpy(), t)
-        print('[bench] TGV %d^3 adjoint %s: %.3f s, J %.10e' % (n, np.dtype(prec).name, t, J[0]), file=sys.stderr,
-              flush=True)
-        del sweep, ctx
-    j32, g32, t32 = res[np.float32]
-    j64, g64, t64 = res[np.float64]
+    for state in ('stage1', 'start'):
+        for prec in (np.float32, np.float64):
+            ctx = case.context(prec)
+            ctx.objective_state = state
+            u0 = torch.as_tensor(u0h.astype(prec), device='cuda').contiguous()
+            if prec == np.float32:
+                t_ = ctx.autotune(u0, reps=3)
+                tuned = tuned or t_
+            sweep = GraphSweep(ctx, dt, interval, obja=1.0)
+            sweep.run(u0, interval)                    # capture + one segment
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            J, ua = sweep.run(u0, n_steps)
+            t = time.perf_counter() - t1
+            res[state, prec] = (float(J[0]), ua.double().cpu().numpy(), t)
+            print('[bench] TGV %d^3 adjoint, objective on %s, %s: %.3f s, J %.10e'
+                  % (n, state, np.dtype(prec).name, t, J[0]), file=sys.stderr, flush=True)
+            del sweep, ctx
+    j32, g32, t32 = res['stage1', np.float32]
+    j64, g64, t64 = res['stage1', np.float64]
+    k32, h32, s32 = res['start', np.float32]
+    k64, h64, s64 = res['start', np.float64]
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     return {'metric': 'cell-updates/s primal+adjoint, FP32 vs FP64, % HBM roofline, L2 err vs FP64',
             'value': nC * n_steps / t32, 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': n_steps, 'warmup': interval,
@@ -293,7 +300,12 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
             'value_fp64': nC * n_steps / t64, 'fp32_over_fp64': t64 / t32,
             'objective_fp32': j32, 'objective_fp64': j64, 'objective_rel_err_fp32_vs_fp64': abs(j32 - j64) / abs(j64),
             'grad_rel_l2_fp32_vs_fp64': {'rho': rel(g32[0:1], g64[0:1]), 'rhoU': rel(g32[1:4], g64[1:4]),
-                                         'rhoE': rel(g32[4:5], g64[4:5])}}
+                                         'rhoE': rel(g32[4:5], g64[4:5])},
+            'objective_on_start_state': {
+                'value': nC * n_steps / s32, 'value_fp64': nC * n_steps / s64, 'fp32_over_fp64': s64 / s32,
+                'objective_fp32': k32, 'objective_fp64': k64, 'objective_rel_err_fp32_vs_fp64': abs(k32 - k64) / abs(k64),
+                'grad_rel_l2_fp32_vs_fp64': {'rho': rel(h32[0:1], h64[0:1]), 'rhoU': rel(h32[1:4], h64[1:4]),
+                                             'rhoE': rel(h32[4:5], h64[4:5])}}}
 
 
 def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, ratio=1.02):

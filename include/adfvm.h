@@ -145,6 +145,8 @@ typedef struct {
 #define ADFVM_OBJ_NONE 0
 #define ADFVM_OBJ_KINETIC_ENERGY 1 /* sum_c 0.5 |rhoU|^2/rho V_c */
 #define ADFVM_OBJ_DRAG 2           /* templates/cylinder_test.py:9-37 on one patch */
+#define ADFVM_OBJSTATE_STAGE1 0    /* objective on the first intermediate stage (density.py:412-413) */
+#define ADFVM_OBJSTATE_START 1     /* objective on the step's start state u_n */
 
 typedef struct {
     int32_t precision;             /* 4 (float kernels) or 8 (double) */
@@ -218,12 +220,27 @@ int adfvm_set_bc_values(adfvm_ctx* ctx, int32_t patch, int32_t slot, const doubl
                         void* stream);
 int adfvm_segments(const adfvm_ctx* ctx);
 
+/* The state a step's objective is evaluated on: ADFVM_OBJSTATE_STAGE1 (the
+ * default) is the reference's hook, RCF.equation evaluating the case's
+ * objective on the first intermediate RK stage (density.py:354-357,412-413);
+ * ADFVM_OBJSTATE_START evaluates it on the step's start state u_n, so that
+ * the time-integrated objective is a sum over the trajectory's states
+ * u_0 .. u_{N-1}.  Applies to obj_out of adfvm_primal_step, to the
+ * objective's adjoint in adfvm_adjoint_step and to the stage at which
+ * adfvm_adjoint_step_cotangent adds obj_grad.  dtc stays on the first
+ * intermediate stage.  No counterpart in the reference (which has the first
+ * form only); the start-state form is the second of the two objective
+ * definitions the 64^3 Taylor-Green adjoint reports. */
+int adfvm_set_objective_state(adfvm_ctx* ctx, int32_t which);
+int adfvm_objective_state(const adfvm_ctx* ctx);
+
 /* One explicit time step in place: u <- step(u).  The reference's primal
  * Function (density.py:97-105: SSP-RK3 via timestep.py:27-55 of
  * RCF.equation, density.py:339-414).  source: device [5][nIC] or NULL.
  * dtc_out (device double, nullable): max_c sum_f (|Un|+a)A/V evaluated on the
  * first intermediate stage, density.py:405-413.  obj_out (device double,
- * nullable): the objective on the same stage, in FP64.  tape (device,
+ * nullable): the objective on the same stage (or on u_n, see
+ * adfvm_set_objective_state), in FP64.  tape (device,
  * adfvm_tape_bytes() bytes, nullable): receives the step's intermediate
  * stage states and stage gradients for a later adfvm_adjoint_step. */
 int adfvm_primal_step(adfvm_ctx* ctx, void* u, double dt, const void* source,
@@ -257,7 +274,8 @@ int adfvm_adjoint_step(adfvm_ctx* ctx, const void* u_n, double dt, const void* s
 /* adfvm_adjoint_step for an objective the caller evaluates itself: obj_grad
  * (device [5][nIC]) is dJ_n/du at the step's objective stage (the first
  * intermediate state, density.py:412-413 -- the stage-1 state a tape holds
- * first, or u_n for a one-stage integrator), added with weight obja to the
+ * first, or u_n for a one-stage integrator or under ADFVM_OBJSTATE_START),
+ * added with weight obja to the
  * adjoint there, besides the context's own objective (if any).  The
  * reference's case-file objective(fields, solver) is differentiated by adpy
  * as part of the primal Function (density.py:354-357, apps/adjoint.py:101);

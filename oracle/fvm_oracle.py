@@ -33,7 +33,9 @@ How each function is pinned (tests/test_oracle.py unless named):
                            the shock tube against the exact solution
                            (tests/golden/shocktube_exact_output.txt, the
                            reference's cases/shockTube/output)
-  objective_value          drag and kinetic energy in closed form
+  objective_value          drag and kinetic energy in closed form; step()'s
+                           objective on the stage-1 or (obj_on_start) the
+                           input state, the latter the closed form again
   face_gradient            its defining property on a sheared (non-orthogonal)
                            mesh: the component along D is the difference
                            quotient, the part normal to D is unchanged
@@ -570,17 +572,22 @@ class Case(object):
             alpha, beta = [[1.]], [1.]
         return alpha, beta
 
-    def step(self, rho, rhoU, rhoE, dt, source=None, ghostInit=None):
+    def step(self, rho, rhoU, rhoE, dt, source=None, ghostInit=None, obj_on_start=False):
         """One time step (timestep.py:27-55): returns (rho', rhoU', rhoE',
         dtc, obj) with dtc and obj taken on the stage-1 state
-        (density.py:412-413)."""
+        (density.py:412-413).  obj_on_start: the objective on the step's
+        start state instead (the library's ADFVM_OBJSTATE_START; no
+        counterpart in the reference)."""
         alpha, beta = self.coefficients()
         fields = [[rho, rhoU, rhoE]]
         dtc = obj = None
+        dtc_stage = 1 if len(beta) > 1 else 0
+        obj_stage = 0 if obj_on_start else dtc_stage
         for i in range(len(beta)):
             (d0, d1, d2, dtcc) = self.equation(*fields[i], ghostInit=ghostInit)
-            if i == 1 or len(beta) == 1:
+            if i == dtc_stage:
                 dtc = dtcc.max()
+            if i == obj_stage:
                 U, T, p = self.primitive(*fields[i])
                 U, T, p = self.ghosts(U, T, p, ghostInit)
                 obj = self.objective_value(*fields[i], U, T, p)

@@ -125,14 +125,17 @@ def test_autograd_functions():
     assert abs(ad - fd) <= 1e-5 * abs(fd)
 
 
-def test_tgv64_adjoint_vs_central_fd():
+@pytest.mark.parametrize('objective_state', ['stage1', 'start'])
+def test_tgv64_adjoint_vs_central_fd(objective_state):
     """Taylor-Green 64^3, FP64: J = sum over 400 steps of the step objective
-    (kinetic energy on the stage-1 state); the reverse sweep checkpoints
-    every 20 steps; the gradient with respect to a seeded N(0,1) initial
-    velocity perturbation (at fixed rho and T) against central differences
-    with eps = 1e-6."""
+    (kinetic energy on the stage-1 state -- the reference's hook -- or on
+    each step's start state u_n); the reverse sweep checkpoints every 20
+    steps; the gradient with respect to a seeded N(0,1) initial velocity
+    perturbation (at fixed rho and T) against central differences with
+    eps = 1e-6."""
     from adfvm_amd import adjoint
     ctx, u0, dt, s = tgv_context(64, np.float64)
+    ctx.objective_state = objective_state
     nSteps = 400
     dts = [dt] * nSteps
     ua = adjoint.sweep(ctx, u0, dts, schedule='uniform', interval=20)
@@ -147,7 +150,8 @@ def test_tgv64_adjoint_vs_central_fd():
     eps = 1e-6
     fd = (J(eps) - J(-eps)) / (2 * eps)
     rel = abs(ad - fd) / abs(fd)
-    print('TGV 64^3 400 steps: dJ/deps adjoint %.12e  central FD %.12e  rel %.2e' % (ad, fd, rel))
+    print('TGV 64^3 400 steps, objective on %s: dJ/deps adjoint %.12e  central FD %.12e  rel %.2e'
+          % (objective_state, ad, fd, rel))
     assert rel <= 1e-4
 
 

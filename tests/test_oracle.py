@@ -354,6 +354,30 @@ def test_objectives_closed_forms():
     assert np.isclose(ke, 0.5 * float(rho[0, 0]) * Ux ** 2 * 0.8 * 0.6 * 0.4, rtol=1e-12)
 
 
+def test_step_objective_state_stage1_and_start():
+    """step()'s objective: by default on the first intermediate stage
+    (density.py:412-413), with obj_on_start on the step's input -- there the
+    kinetic energy of a uniform flow in closed form, while the stage-1 value
+    differs once the flow is not steady (and dtc is the same either way)."""
+    patches = {'x0': ('inlet', 'patch'), 'x1': ('outlet', 'patch'), 'y0': ('wall', 'patch'),
+               'y1': ('top', 'patch'), 'z0': ('sides', 'empty'), 'z1': ('sides', 'empty')}
+    md = meshgen.box((4, 3, 2), (0.8, 0.6, 0.4), patches=patches)
+    bcs = {'wall': {'U': {'type': 'fixedValue', 'value': np.zeros(3)}}}
+    c, om = oracle(md, bcs, mu=2e-5, objective=('kinetic_energy',))
+    nIC = om.nInternalCells
+    Ux = 12.0
+    U = T_(np.tile([Ux, 0., 0.], (nIC, 1)))
+    rho, rhoU, rhoE = c.conservative(U, T_(np.full((nIC, 1), 300.)), T_(np.full((nIC, 1), 1e5)))
+    dt = 1e-5
+    *_, dtc0, obj0 = c.step(rho, rhoU, rhoE, dt, obj_on_start=True)
+    *_, dtc1, obj1 = c.step(rho, rhoU, rhoE, dt)
+    ke = 0.5 * float(rho[0, 0]) * Ux ** 2 * 0.8 * 0.6 * 0.4
+    assert np.isclose(float(obj0), ke, rtol=1e-12)
+    # the no-slip wall decelerates the first stage: its kinetic energy is lower
+    assert float(obj1) < ke * (1 - 1e-9)
+    assert float(dtc0) == float(dtc1)
+
+
 # ------------------------------------------------------------ a14 (reverse) -
 def test_oracle_step_gradient_dot_product_vs_fd():
     """torch autograd of the oracle step is the exact discrete adjoint:
