@@ -43,6 +43,7 @@ class GraphSweep(object):
         self.J = torch.zeros(nseg, dtype=torch.float64, device='cuda')
         self.bc_grad = ctx.bc_state() if bc_grad else None
         self._fwd = self._rev = None
+        self._captured_for = None        # the context's objective state the graphs were captured with
         self.keep_final = False          # set to keep the state after the forward sweep (u_final)
         self.u_final = None
 
@@ -140,6 +141,13 @@ class GraphSweep(object):
         run)."""
         if n_steps % self.interval:
             raise ValueError('n_steps must be a multiple of interval')
+        # a graph replays the launches it recorded: captured graphs are
+        # dropped when the context's objective state (the stage whose state
+        # the objective reads) has changed since
+        state = self.ctx.objective_state
+        if self._captured_for != state:
+            self._fwd = self._rev = self._whole = None
+            self._captured_for = state
         if self.whole:
             if not self.graphs:
                 raise ValueError('whole=True captures the sweep as one graph: it needs graphs=True')

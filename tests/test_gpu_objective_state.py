@@ -162,3 +162,26 @@ def test_switching_back_restores_the_reference_hook_bitwise():
             c.adjoint_step(ud, dt, ua, obja=1.0)
             res.append((obj.item(), ua.cpu().numpy()))
         assert res[0][0] == res[1][0] and np.array_equal(res[0][1], res[1][1])
+
+
+def test_graph_sweep_follows_the_objective_state():
+    """A GraphSweep captured with the stage-1 objective, replayed after the
+    context switched to the start-state objective, recaptures: its result
+    equals an eager sweep in the new state bitwise."""
+    from adfvm_amd.graphs import GraphSweep
+    case, params = cf.tgv(8, seed=5)
+    u = cf.tgv_state(case, params)
+    u[1:4] += u[0] * np.random.default_rng(8).normal(size=u[1:4].shape) * 0.3
+    ctx = case.context(np.float64)
+    u0 = torch.as_tensor(u, device='cuda').contiguous()
+    dt, interval, n = 2e-4, 2, 4
+    sweep = GraphSweep(ctx, dt, interval, obja=1.0)
+    J1, _ = sweep.run(u0, n)
+    ctx.objective_state = 'start'
+    J2, ua2 = sweep.run(u0, n)
+    ua2 = ua2.clone()
+    eager = GraphSweep(ctx, dt, interval, obja=1.0, graphs=False)
+    J3, ua3 = eager.run(u0, n)
+    print('graph sweep objective: stage1 %.15e start %.15e eager start %.15e' % (J1[0], J2[0], J3[0]))
+    assert J1[0] != J2[0]
+    assert J2[0] == J3[0] and torch.equal(ua2, ua3)
