@@ -1,9 +1,13 @@
 """OpenFOAM case I/O: constant/polyMesh and volScalar/volVector fields.
 
 Reads ASCII and binary files (the reference reads with adFVM/mesh.py:206-235
-and adFVM/field.py:198-264); writes ASCII.  Only what the solver path needs:
-points, faces (quads), owner, neighbour, boundary; internalField
-uniform/nonuniform and boundaryField dictionaries.
+and adFVM/field.py:198-264).  Writes the polyMesh in ASCII and fields in
+config.fileFormat, binary by default as the reference's
+(adFVM/config.py:198, IOField.writeFoamField field.py:436-476 with
+writeField mesh.py:1048-1059: a nonuniform list is its count, '(', the
+float64 values' bytes and ')').  Only what the solver path needs: points,
+faces (quads), owner, neighbour, boundary; internalField uniform/nonuniform
+and boundaryField dictionaries.
 """
 import os
 import re
@@ -22,7 +26,7 @@ HEADER = '''/*--------------------------------*- C++ -*-------------------------
 FoamFile
 {
     version     2.0;
-    format      ascii;
+    format      %s;
     class       %s;
     location    "%s";
     object      %s;
@@ -137,7 +141,7 @@ def write_polymesh(md, caseDir):
 
     def w(name, cls, body):
         with open(os.path.join(pm, name), 'w') as f:
-            f.write(HEADER % (cls, 'constant/polyMesh', name))
+            f.write(HEADER % ('ascii', cls, 'constant/polyMesh', name))
             f.write(body)
 
     pts = '\n'.join('(%.17g %.17g %.17g)' % tuple(p) for p in md.points)
@@ -193,20 +197,62 @@ def _field_values(text, n, ncomp, binary):
     return vals, rest
 
 
+_TOKEN = re.compile(rb'\s*(?://[^\n]*\s*|/\*.*?\*/\s*)*([A-Za-z0-9_.\-+<>]+|[{}])', re.S)
+_BIN_LIST = re.compile(rb'\s*nonuniform\s+List<(\w+)>\s*(\d+)\s*\(')
+
+
+def _boundary_field(text, start, binary):
+    """{patch: {key: value}} of the boundaryField dictionary opening at
+    text[start] ('{').  Values are stripped strings, except a binary file's
+    nonuniform lists, read by their byte count (8 per scalar component: the
+    bytes may hold any character) as (count, components) float64 arrays."""
+    out = {}
+    i = start + 1
+    while True:
+        m = _TOKEN.match(text, i)
+        if m is None or m.group(1) == b'}':
+            return out
+        name, i = m.group(1).decode(), m.end()
+        m = _TOKEN.match(text, i)
+        if m is None or m.group(1) != b'{':
+            raise ValueError('boundaryField: no dictionary for patch %s' % name)
+        i = m.end()
+        d = {}
+        while True:
+            m = _TOKEN.match(text, i)
+            if m is None:
+                raise ValueError('boundaryField: patch %s not closed' % name)
+            i = m.end()
+            if m.group(1) == b'}':
+                break
+            key = m.group(1).decode()
+            mb = _BIN_LIST.match(text, i) if binary else None
+            if mb:
+                k = 3 if mb.group(1) == b'vector' else 1
+                cnt = int(mb.group(2))
+                s = mb.end()
+                d[key] = np.frombuffer(text[s:s + 8 * k * cnt], np.float64).reshape(cnt, k).copy()
+                e = re.compile(rb'\s*\)\s*;').match(text, s + 8 * k * cnt)
+                if e is None:
+                    raise ValueError('boundaryField: %s.%s holds fewer than %d values' % (name, key, cnt))
+                i = e.end()
+            else:
+                e = text.index(b';', i)
+                d[key] = text[i:e].decode('latin-1').strip()
+                i = e + 1
+        out[name] = d
+
+
 def read_field(caseDir, t, name, n, ncomp):
-    """(internal values (n, ncomp), boundaryField {patch: {key: value-str}})."""
+    """(internal values (n, ncomp), boundaryField {patch: {key: value}}):
+    values as strings ('uniform 300', 'zeroGradient', an ASCII nonuniform
+    list), a binary file's nonuniform lists as float64 arrays."""
     raw = open(os.path.join(time_dir(caseDir, t), name), 'rb').read()
     hdr, body = _header(raw)
-    vals, rest = _field_values(body, n, ncomp, hdr.get('format') == 'binary')
+    binary = hdr.get('format') == 'binary'
+    vals, rest = _field_values(body, n, ncomp, binary)
     m = re.search(rb'boundaryField\s*\{', rest)
-    bnd = {}
-    if m:
-        text = _strip_comments(rest[m.end():]).decode('latin-1')
-        for pname, inner in re.findall(r'([A-Za-z0-9_]+)\s*\{([^{}]*)\}', text):
-            d = {}
-            for k, v in re.findall(r'(\w+)\s+([^;]+);', inner):
-                d[k] = v.strip()
-            bnd[pname] = d
+    bnd = _boundary_field(rest, m.end() - 1, binary) if m else {}
     return vals, bnd
 
 
@@ -218,42 +264,61 @@ def read_fields(mesh, t, suffix=''):
     return U, T, p, {'U': bU, 'T': bT, 'p': bp}
 
 
-def _fmt_value(v, ncomp):
+def _fmt_value(v, ncomp, binary=False):
+    """A field or boundary value as it follows its keyword, as bytes: uniform
+    when every row is equal, else a nonuniform list -- in a binary file the
+    count, '(', the values' float64 bytes and ')' (adFVM/mesh.py:1048-1059)."""
     a = np.asarray(v, np.float64).reshape(-1, ncomp)
-    if (a == a[0]).all():
-        return 'uniform ' + ('%.17g' % a[0, 0] if ncomp == 1 else '(%s)' % ' '.join('%.17g' % x for x in a[0]))
+    if len(a) and (a == a[0]).all():
+        return ('uniform ' + ('%.17g' % a[0, 0] if ncomp == 1 else
+                              '(%s)' % ' '.join('%.17g' % x for x in a[0]))).encode()
     cls = 'scalar' if ncomp == 1 else 'vector'
+    if binary:
+        return b'nonuniform List<%s>\n%d\n(' % (cls.encode(), len(a)) + np.ascontiguousarray(a).tobytes() + b')\n'
+    # ASCII: the list closes on its own line; the caller adds ';'
     body = '\n'.join(('%.17g' % r[0]) if ncomp == 1 else '(%s)' % ' '.join('%.17g' % x for x in r) for r in a)
-    return 'nonuniform List<%s> %d\n(\n%s\n)\n' % (cls, len(a), body)
+    return ('nonuniform List<%s> %d\n(\n%s\n)\n' % (cls, len(a), body)).encode()
 
 
-def write_fields(mesh, t, fields, boundaries, default=None):
+def write_fields(mesh, t, fields, boundaries, default=None, fmt=None):
     """Write each field of {name: (nIC, k) array} to the time directory;
     a patch's boundaryField entry is boundaries[name][patch], else
-    default[patch] (the mesh's calculated boundary by default)."""
+    default[patch] (the mesh's calculated boundary by default).  fmt:
+    'binary' or 'ascii' (config.fileFormat when None)."""
+    from . import config
+    fmt = fmt or config.fileFormat
+    if fmt not in ('binary', 'ascii'):
+        raise ValueError('file format %r (binary, ascii)' % (fmt,))
+    binary = fmt == 'binary'
     d = time_dir(mesh.caseDir, t)
     os.makedirs(d, exist_ok=True)
     for name, arr in fields.items():
         arr = np.asarray(arr, np.float64)
         ncomp = arr.shape[1] if arr.ndim == 2 else 1
         cls = 'volScalarField' if ncomp == 1 else 'volVectorField'
-        lines = ['dimensions      [0 0 0 0 0 0 0];', '', 'internalField   %s;' % _fmt_value(arr, ncomp).rstrip(),
-                 '', 'boundaryField', '{']
+        def val(a, k):
+            # a binary list keeps ')\n' before ';', as the reference writes it
+            # (and its bytes are never stripped)
+            b = _fmt_value(a, k, binary)
+            return b if binary and b.endswith(b')\n') else b.rstrip()
+
+        out = [b'dimensions      [0 0 0 0 0 0 0];\n\n',
+               b'internalField   ' + val(arr, ncomp) + b';\n\n', b'boundaryField\n{\n']
         specs = boundaries.get(name, {})
         for patch in mesh.boundary:
             spec = specs.get(patch, (default or mesh.calculatedBoundary)[patch])
-            lines.append('    %s\n    {' % patch)
-            lines.append('        type            %s;' % spec['type'])
+            out.append(b'    %s\n    {\n' % patch.encode())
+            out.append(b'        type            %s;\n' % spec['type'].encode())
             for k, v in spec.items():
                 if k == 'type':
                     continue
                 if isinstance(v, str):
-                    lines.append('        %-15s %s;' % (k, v))
+                    out.append(b'        %-15s %s;\n' % (k.encode(), v.encode()))
                 else:
                     kk = 3 if k in ('U0', 'direction') or (k == 'value' and name == 'U') else 1
-                    lines.append('        %-15s %s;' % (k, _fmt_value(v, kk).rstrip()))
-            lines.append('    }')
-        lines.append('}')
-        with open(os.path.join(d, name), 'w') as f:
-            f.write(HEADER % (cls, os.path.basename(d), name))
-            f.write('\n'.join(lines) + '\n')
+                    out.append(b'        %-15s ' % k.encode() + val(v, kk) + b';\n')
+            out.append(b'    }\n')
+        out.append(b'}\n')
+        with open(os.path.join(d, name), 'wb') as f:
+            f.write((HEADER % (fmt, cls, os.path.basename(d), name)).encode())
+            f.write(b''.join(out))

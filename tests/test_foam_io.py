@@ -1,6 +1,7 @@
 """OpenFOAM case I/O (adfvm_amd.foam): the shockTube case files of the
 reference (binary fields, boundary dictionary) and write/read round trips."""
 import os
+import re
 import shutil
 
 import numpy as np
@@ -84,3 +85,62 @@ def test_field_round_trip(tmp_path):
 def test_time_string():
     assert foam.time_string(0) == '0' and foam.time_string(2.0) == '2'
     assert foam.time_string(0.006341272) == '0.00634127200'
+
+
+def _case_with_fields(tmp_path, fmt):
+    md = meshgen.box((3, 2, 2))
+    d = str(tmp_path / ('case_' + fmt))
+    foam.write_polymesh(md, d)
+    mesh = Mesh.create(d)
+    nW = mesh.boundary['walls']['nFaces']
+    rng = np.random.default_rng(1)
+    U = rng.standard_normal((12, 3))
+    p = 1e5 + rng.standard_normal((12, 1))
+    # boundary values whose float64 bytes spell ';', '}', ')' and '{': the
+    # binary reader must count bytes, not scan for delimiters
+    tricky = np.frombuffer(b';;;;;;;;}}}}}}}}))))))))' + b'{' * 8, np.float64)
+    Uw = rng.standard_normal((nW, 3))
+    Uw.ravel()[:4] = tricky
+    pw = 1e5 + rng.standard_normal((nW, 1))
+    bnd = {'U': {'walls': {'type': 'fixedValue', 'value': Uw}},
+           'p': {'walls': {'type': 'fixedValue', 'value': pw}}}
+    foam.write_fields(mesh, 0.5, {'U': U, 'p': p}, bnd, fmt=fmt)
+    return mesh, d, U, p, Uw, pw
+
+
+def test_field_round_trip_binary_and_ascii_nonuniform_boundaries(tmp_path):
+    for fmt in ('binary', 'ascii'):
+        mesh, d, U, p, Uw, pw = _case_with_fields(tmp_path, fmt)
+        U2, bU = foam.read_field(d, 0.5, 'U', 12, 3)
+        p2, bp = foam.read_field(d, 0.5, 'p', 12, 1)
+        assert np.array_equal(U2, U) and np.array_equal(p2, p)
+        from adfvm_amd import BCs
+        nW = mesh.boundary['walls']['nFaces']
+        assert np.array_equal(BCs.parse_value(bU['walls']['value'], nW, 3), Uw)
+        assert np.array_equal(BCs.parse_value(bp['walls']['value'], nW, 1), pw)
+        assert bU['walls']['type'] == 'fixedValue'
+        raw = open(os.path.join(d, '0.50000000000', 'p'), 'rb').read()
+        assert re.search(rb'format\s+%s;' % fmt.encode(), raw)
+
+
+def test_binary_fields_follow_the_reference_layout(tmp_path):
+    """A binary field as adFVM/mesh.py:1048-1059 writes one: 'internalField
+    nonuniform List<scalar>', the count, '(' and the float64 bytes, ')', ';'
+    -- and read back by the reference's own rules (IOField.readFoam,
+    field.py:198-249, with extractField, mesh.py:1021-1046: the values
+    between the first '(' after internalField and the last ')' before
+    boundaryField; a patch's binary value by its byte count)."""
+    mesh, d, U, p, Uw, pw = _case_with_fields(tmp_path, 'binary')
+    raw = open(os.path.join(d, '0.50000000000', 'p'), 'rb').read()
+    assert b'internalField   nonuniform List<scalar>\n12\n(' + p.tobytes() + b')\n;' in raw
+    start_boundary = raw.find(b'boundaryField')
+    data = raw[raw.find(b'internalField'):start_boundary]
+    vals = np.frombuffer(data[data.find(b'(') + 1:data.rfind(b')')], np.float64)
+    assert np.array_equal(vals, p.ravel())
+    rawU = open(os.path.join(d, '0.50000000000', 'U'), 'rb').read()
+    content = rawU[rawU.find(b'boundaryField'):]
+    m = re.search(rb'[ ]+(nonuniform[ ]+List<[a-z]+>[\s\r\n\t0-9]*\()', content[content.find(b'value'):])
+    s = content.find(b'value') + m.end()
+    nbytes = 24 * mesh.boundary['walls']['nFaces']
+    assert np.array_equal(np.frombuffer(content[s:s + nbytes], np.float64).reshape(-1, 3), Uw)
+    assert re.match(rb'\)[\s\r\n\t]*;', content[s + nbytes:])
