@@ -57,8 +57,13 @@ __device__ __forceinline__ int xcd_chunk_block() {
 }
 
 // ------------------------------------------------------------- tables ----
-// slot flags: bit 0 = this cell owns the face; bits 1-2 = slot kind
+// slot flags: bit 0 = this cell owns the face; bits 1-2 = slot kind; bit 3
+// (boundary slots) = the ghost's values are the owner's (zeroGradient U, and
+// zeroGradient or symmetry T and p: BCs.py:97-104,120-137), so neither the
+// ghost rule nor its adjoint needs the patch
 enum { SLOT_INTERNAL = 0, SLOT_COUPLED = 1, SLOT_CHARACTERISTIC = 2, SLOT_BOUNDARY = 3 };
+constexpr int SLOT_GHOST_COPY = 8;
+__host__ __device__ __forceinline__ int slot_kind(int flags) { return (flags >> 1) & 3; }
 
 // face geometry rows (each nFaces long)
 enum { G_A = 0, G_NX, G_NY, G_NZ, G_W, G_DELTA, G_DX, G_DY, G_DZ, G_LW0, G_LW1,
@@ -281,11 +286,13 @@ __global__ void __launch_bounds__(BS) grad_kernel(DMesh<T> m, Phys<T> ph, const 
         const int o = ld<ST>(m.sOther + j * nIC + c);
         T wo[5];
         if constexpr (GH) {
-            // a ghost slot's value from this cell (the face's owner)
+            // a ghost slot's value from this cell (the face's owner): a copy
+            // without reading the patch when the slot is flagged so
+            const bool copy = ld<ST>(m.sFlags + j * nIC + c) & SLOT_GHOST_COPY;
             T wi[5], wg[5];
             load_prim(ph, u, nIC, u, nIC, o < 0 ? c : o, wi);
             for (int i = 0; i < 5; i++) wg[i] = wc[i];
-            if (o < 0) ghost_value(m, ph, -1 - o, wc, wg);
+            if (o < 0 && !copy) ghost_value(m, ph, -1 - o, wc, wg);
             for (int i = 0; i < 5; i++) wo[i] = o < 0 ? wg[i] : wi[i];
         } else {
             load_prim(ph, u, nIC, ghost, m.nBF, o, wo);
@@ -1194,7 +1201,14 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
             }
             sFace[j * nIC + cc] = f;
             sOther[j * nIC + cc] = other;
-            sFlags[j * nIC + cc] = (unsigned char)((own ? 1 : 0) | (kind << 1));
+            bool copy = false;
+            if (kind == SLOT_BOUNDARY) {
+                const adfvm_patch_desc& Pd = d->patches[bPatch[f - nIF]];
+                auto scalarCopy = [](int bc) { return bc == ADFVM_BC_ZERO_GRADIENT || bc == ADFVM_BC_SYMMETRY; };
+                copy = Pd.cbc == ADFVM_CBC_NONE && Pd.bcU == ADFVM_BC_ZERO_GRADIENT && scalarCopy(Pd.bcT) &&
+                       scalarCopy(Pd.bcp);
+            }
+            sFlags[j * nIC + cc] = (unsigned char)((own ? 1 : 0) | (kind << 1) | (copy ? SLOT_GHOST_COPY : 0));
             cellNb[j * nIC + cc] = d->cellNeighbours[6 * cc + j];
         }
     }
@@ -1230,7 +1244,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
             int& slot = (fl & 1) ? posP[f] : posN[f];
             if (slot != -1) return bad("face in two slots of the same side");
             slot = (int)(j * nIC + cc);
-            if ((fl >> 1) == SLOT_COUPLED) e++;
+            if (slot_kind(fl) == SLOT_COUPLED) e++;
         }
         maxExtra = std::max(maxExtra, e);
     }
@@ -1238,7 +1252,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     for (long cc = 0; cc < nIC; cc++) {
         int e = 0;
         for (int j = 0; j < 6; j++) {
-            if ((sFlags[j * nIC + cc] >> 1) != SLOT_COUPLED) continue;
+            if (slot_kind(sFlags[j * nIC + cc]) != SLOT_COUPLED) continue;
             const int g = sPair[j * nIC + cc];
             if (posN[g] != -1) return bad("face with two neighbour-side slots");
             posN[g] = (int)((6 + e++) * nIC + cc);
@@ -1271,7 +1285,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
         bool more = true;              // still in the leading run of owned internal slots
         for (int j = 0; j < 6; j++) {
             const int fl = sFlags[j * nIC + cc];
-            const bool ownInt = (fl & 1) && (fl >> 1) == SLOT_INTERNAL;
+            const bool ownInt = (fl & 1) && slot_kind(fl) == SLOT_INTERNAL;
             if (ownInt && (!more || sFace[j * nIC + cc] <= prev)) { c->ownerSum = false; break; }
             if (ownInt) prev = sFace[j * nIC + cc];
             else more = false;
