@@ -15,6 +15,7 @@
 //    block folds the partials in index order, in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -64,6 +65,17 @@ __device__ __forceinline__ int xcd_chunk_block() {
 enum { SLOT_INTERNAL = 0, SLOT_COUPLED = 1, SLOT_CHARACTERISTIC = 2, SLOT_BOUNDARY = 3 };
 constexpr int SLOT_GHOST_COPY = 8;
 __host__ __device__ __forceinline__ int slot_kind(int flags) { return (flags >> 1) & 3; }
+// face kind bytes (device copy): FACE_* in bits 0-1, bit 2 for a boundary
+// face whose ghost copies its owner (as the slot flag above)
+constexpr int FACE_GHOST_COPY = 4;
+__host__ __device__ __forceinline__ int face_kind(int k) { return k & 3; }
+
+// the ghost of a non-characteristic patch is its owner's values
+inline bool ghost_copies_owner(const adfvm_patch_desc& Pd) {
+    auto scalarCopy = [](int bc) { return bc == ADFVM_BC_ZERO_GRADIENT || bc == ADFVM_BC_SYMMETRY; };
+    return Pd.fluxKind == ADFVM_FLUX_BOUNDARY && Pd.cbc == ADFVM_CBC_NONE && Pd.bcU == ADFVM_BC_ZERO_GRADIENT &&
+           scalarCopy(Pd.bcT) && scalarCopy(Pd.bcp);
+}
 
 // face geometry rows (each nFaces long)
 enum { G_A = 0, G_NX, G_NY, G_NZ, G_W, G_DELTA, G_DX, G_DY, G_DZ, G_LW0, G_LW1,
@@ -375,18 +387,20 @@ __global__ void __launch_bounds__(BS) face_flux_kernel(DMesh<T> m, Phys<T> ph, c
     const long nIC = m.nIC, nBF = m.nBF, nF = m.nF;
     const int P = m.owner[f];
     const int o = ld<ST>(m.fOther + f);
-    const int kind = ld<ST>(m.fKind + f);
+    const int kindByte = ld<ST>(m.fKind + f);
+    const int kind = face_kind(kindByte);
     FaceGeo<T> fg;
     load_geo<ST>(m, f, fg);
     T wP[5], gP[GREC], wN[5], gN[GREC], F[5], lam;
     load_prim(ph, u, nIC, ghost, nBF, P, wP);
     load_grad(grad, nIC, P, gP);
     if constexpr (GH) {
-        // a boundary face's ghost from its owner
+        // a boundary face's ghost from its owner (a copy without reading the
+        // patch when the face is flagged so)
         T wi[5], wg[5];
         load_prim(ph, u, nIC, u, nIC, o < 0 ? P : o, wi);
         for (int i = 0; i < 5; i++) wg[i] = wP[i];
-        if (o < 0) ghost_value(m, ph, -1 - o, wP, wg);
+        if (o < 0 && !(kindByte & FACE_GHOST_COPY)) ghost_value(m, ph, -1 - o, wP, wg);
         for (int i = 0; i < 5; i++) wN[i] = o < 0 ? wg[i] : wi[i];
     } else {
         load_prim(ph, u, nIC, ghost, nBF, o, wN);
@@ -735,8 +749,11 @@ struct adfvm_ctx {
     // internal faces sorted by owner: per block size, the face ranges of the
     // owner-summing face adjoint blocks (no owner split between two ranges)
     // boundary faces with a ghost by face kind (0 characteristic, 1 boundary)
-    int* kindFaces[2] = {nullptr, nullptr};
-    int nKindFaces[2] = {0, 0};
+    // boundary faces with a ghost by face kind (characteristic, boundary):
+    // per block of BLOCK threads its first boundary index and face count,
+    // blocks never crossing a gap between runs of one kind
+    int* kindBlk[3] = {nullptr, nullptr, nullptr};
+    int nKindBlk[3] = {0, 0, 0};
     bool ownerSum = false;
     // the streaming policy (ld / st): non-temporal single-use rows when one
     // stage's working set exceeds the Infinity Cache (adfvm_set_streaming)
@@ -1201,13 +1218,7 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
             }
             sFace[j * nIC + cc] = f;
             sOther[j * nIC + cc] = other;
-            bool copy = false;
-            if (kind == SLOT_BOUNDARY) {
-                const adfvm_patch_desc& Pd = d->patches[bPatch[f - nIF]];
-                auto scalarCopy = [](int bc) { return bc == ADFVM_BC_ZERO_GRADIENT || bc == ADFVM_BC_SYMMETRY; };
-                copy = Pd.cbc == ADFVM_CBC_NONE && Pd.bcU == ADFVM_BC_ZERO_GRADIENT && scalarCopy(Pd.bcT) &&
-                       scalarCopy(Pd.bcp);
-            }
+            const bool copy = kind == SLOT_BOUNDARY && ghost_copies_owner(d->patches[bPatch[f - nIF]]);
             sFlags[j * nIC + cc] = (unsigned char)((own ? 1 : 0) | (kind << 1) | (copy ? SLOT_GHOST_COPY : 0));
             cellNb[j * nIC + cc] = d->cellNeighbours[6 * cc + j];
         }
@@ -1264,13 +1275,27 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     }
     c->nPlanes = 6 + maxExtra;
     int rc;
-    for (int k = 0; k < 2; k++) {
-        const int kind = k == 0 ? FACE_CHARACTERISTIC : FACE_BOUNDARY;
-        std::vector<int> list;
-        for (long b = 0; b < nBF; b++)
-            if (fKind[nIF + b] == kind) list.push_back((int)b);
-        c->nKindFaces[k] = (int)list.size();
-        if ((rc = upload(c, (void**)&c->kindFaces[k], list)) != ADFVM_OK) return rc;
+    // the three groups: characteristic faces, boundary-kind faces whose
+    // ghost reads its patch, boundary-kind faces whose ghost copies the owner
+    std::vector<int> group(nBF, -1);
+    for (long b = 0; b < nBF; b++) {
+        if (fKind[nIF + b] == FACE_CHARACTERISTIC) group[b] = 0;
+        else if (fKind[nIF + b] == FACE_BOUNDARY) group[b] = ghost_copies_owner(d->patches[bPatch[b]]) ? 2 : 1;
+    }
+    for (int k = 0; k < 3; k++) {
+        std::vector<int> blk;                   // (first boundary index, count) per block
+        for (long b = 0; b < nBF;) {
+            if (group[b] != k) { b++; continue; }
+            long e = b;
+            while (e < nBF && group[e] == k) e++;
+            for (long b0 = b; b0 < e; b0 += BLOCK) {
+                blk.push_back((int)b0);
+                blk.push_back((int)std::min<long>(BLOCK, e - b0));
+            }
+            b = e;
+        }
+        c->nKindBlk[k] = (int)(blk.size() / 2);
+        if ((rc = upload(c, (void**)&c->kindBlk[k], blk)) != ADFVM_OK) return rc;
     }
     // The face adjoint sums a cell's owner sides in LDS when one owner's
     // internal faces are consecutive (owners non-decreasing over the internal
@@ -1308,7 +1333,12 @@ extern "C" int adfvm_ctx_create(const adfvm_solver_desc* d, const adfvm_physics*
     if ((rc = upload(c, (void**)&c->posN, posN)) != ADFVM_OK) return rc;
     std::vector<int> owner(d->owner, d->owner + nF), nbFull(d->neighbour, d->neighbour + nF);
     if ((rc = upload(c, (void**)&c->fOther, fOther)) != ADFVM_OK) return rc;
-    if ((rc = upload(c, (void**)&c->fKind, fKind)) != ADFVM_OK) return rc;
+    {
+        std::vector<unsigned char> fKindDev(fKind);
+        for (long b = 0; b < nBF; b++)
+            if (ghost_copies_owner(d->patches[bPatch[b]])) fKindDev[nIF + b] |= FACE_GHOST_COPY;
+        if ((rc = upload(c, (void**)&c->fKind, fKindDev)) != ADFVM_OK) return rc;
+    }
     if ((rc = upload(c, (void**)&c->sFace, sFace)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->sOther, sOther)) != ADFVM_OK) return rc;
     if ((rc = upload(c, (void**)&c->sFlags, sFlags)) != ADFVM_OK) return rc;
@@ -1335,8 +1365,8 @@ extern "C" void adfvm_ctx_destroy(adfvm_ctx* c) {
                     c->grad2, c->adjAcc, c->adjG,
                     c->partial, c->scal, c->fOther, c->fKind, c->fbuf, c->frec,
                     c->segCell, c->segBlock, c->blockSeg, c->posP, c->posN, c->sgeo, c->ngeo, c->visc,
-                    c->fvjpBlk[0], c->fvjpBlk[1], c->fvjpBlk[2], c->fvjpBlk[3], c->kindFaces[0],
-                    c->kindFaces[1]};
+                    c->fvjpBlk[0], c->fvjpBlk[1], c->fvjpBlk[2], c->fvjpBlk[3], c->kindBlk[0],
+                    c->kindBlk[1], c->kindBlk[2]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
