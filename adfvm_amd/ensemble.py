@@ -69,16 +69,20 @@ class TaylorGreenEnsemble(object):
     def n_cells(self):
         return self.mesh.nInternalCells
 
-    def run(self, n_steps=200, interval=20, graphs=True, tapes=True, whole=None):
+    def run(self, n_steps=200, interval=20, graphs=True, tapes=True, whole=None, overlap=None):
         """Primal + adjoint of J_k = sum over steps of instance k's kinetic
         energy (on each step's stage-1 state).  Returns (J (nSeg,) FP64
         host array, ua: d(sum_k J_k)/du0 on the device, i.e. each instance's
         own gradient).  With graphs, the whole sweep is one captured HIP
         graph, a single launch per run (whole=False: one graph per
-        checkpoint segment, replayed per segment)."""
+        checkpoint segment, replayed per segment).  overlap: a segment's
+        re-run alongside the next segment's adjoint steps (graphs.GraphSweep;
+        None: when a second set of segment buffers fits)."""
         from .graphs import GraphSweep
         whole = graphs if whole is None else whole
-        key = (interval, graphs, tapes, whole)
+        key = (interval, graphs, tapes, whole, overlap)
         if key not in self._sweeps:                 # captured once, replayed by later runs
-            self._sweeps[key] = GraphSweep(self.ctx, self.dt, interval, 1.0, graphs, tapes, whole=whole)
-        return self._sweeps[key].run(self.u0, n_steps)
+            self._sweeps[key] = GraphSweep(self.ctx, self.dt, interval, 1.0, graphs, tapes, whole=whole,
+                                           overlap=overlap)
+        self.last_sweep = self._sweeps[key]
+        return self.last_sweep.run(self.u0, n_steps)

@@ -213,7 +213,7 @@ def cpu_baseline(mesh, mu0, u0, dt, steps=8):
                 finite=bool(np.isfinite(ua).all() and np.isfinite(u).all()))
 
 
-def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
+def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0, overlap=None):
     """Batched ensemble: n_instances Taylor-Green boxes, sizes drawn by seed
     from {32, 48, 64, 96}^3, +-1% seeded velocity perturbation, advanced
     primal + adjoint as one concatenated mesh, the whole sweep one captured
@@ -225,10 +225,10 @@ def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
                                                                            time.time() - t0),
           file=sys.stderr, flush=True)
     tuned = ens.ctx.autotune(ens.u0, reps=2)
-    ens.run(n_steps=n_steps, interval=interval)                   # warm-up: captures the sweep's graph, runs it
+    ens.run(n_steps=n_steps, interval=interval, overlap=overlap)  # warm-up: captures the sweep's graph, runs it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J, ua = ens.run(n_steps=n_steps, interval=interval)
+    J, ua = ens.run(n_steps=n_steps, interval=interval, overlap=overlap)
     t = time.perf_counter() - t0
     ok = bool(np.isfinite(J).all()) and bool(torch.isfinite(ua).all())
     value = ens.n_cells * n_steps / t
@@ -243,10 +243,11 @@ def ensemble_bench(torch, n_instances=64, n_steps=200, interval=20, seed=0):
                        'size_counts': {str(k): ens.sizes.count(k) for k in sorted(set(ens.sizes))},
                        'dt': ens.dt},
             'block_sizes': {k: v[0] for k, v in tuned.items()}, 'finite': ok,
+            'overlapped_reverse': ens.last_sweep.overlap,
             'objective_range': [float(J.min()), float(J.max())]}
 
 
-def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
+def adjoint64_bench(torch, n=64, n_steps=400, interval=20, overlap=None):
     """Taylor-Green n^3 adjoint: n_steps fixed-dt steps (CFL 0.5) primal
     plus the adjoint of the kinetic energy summed over the steps (the
     reference's objective hook on the stage-1 state), uniform checkpoints
@@ -274,13 +275,14 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
             if prec == np.float32:
                 t_ = ctx.autotune(u0, reps=3)
                 tuned = tuned or t_
-            sweep = GraphSweep(ctx, dt, interval, obja=1.0)
+            sweep = GraphSweep(ctx, dt, interval, obja=1.0, overlap=overlap)
             sweep.run(u0, interval)                    # capture + one segment
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             J, ua = sweep.run(u0, n_steps)
             t = time.perf_counter() - t1
             res[state, prec] = (float(J[0]), ua.double().cpu().numpy(), t)
+            ovl = sweep.overlap
             print('[bench] TGV %d^3 adjoint, objective on %s, %s: %.3f s, J %.10e'
                   % (n, state, np.dtype(prec).name, t, J[0]), file=sys.stderr, flush=True)
             del sweep, ctx
@@ -296,7 +298,7 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
             'config': {'workload': 'TGV %d^3 periodic hex, random permutation + RCM, %d steps primal + adjoint of '
                                    'the summed kinetic energy, checkpoints every %d steps, HIP-graph segments'
                                    % (n, n_steps, interval), 'n_cells': nC, 'dt': dt},
-            'block_sizes': {k: v[0] for k, v in tuned.items()},
+            'block_sizes': {k: v[0] for k, v in tuned.items()}, 'overlapped_reverse': ovl,
             'value_fp64': nC * n_steps / t64, 'fp32_over_fp64': t64 / t32,
             'objective_fp32': j32, 'objective_fp64': j64, 'objective_rel_err_fp32_vs_fp64': abs(j32 - j64) / abs(j64),
             'grad_rel_l2_fp32_vs_fp64': {'rho': rel(g32[0:1], g64[0:1]), 'rhoU': rel(g32[1:4], g64[1:4]),
@@ -308,7 +310,7 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20):
                                              'rhoE': rel(h32[4:5], h64[4:5])}}}
 
 
-def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, ratio=1.02):
+def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, ratio=1.02, overlap=None):
     """2-D cylinder wake: O-grid of n_theta x n_r x 1 hex cells, radial
     spacing growing by `ratio` from r = 0.5 D to 20 D, Mach 0.2, Re = 100,
     isothermal no-slip wall, CBC_UPT far field, Lax-Friedrichs at the
@@ -334,13 +336,14 @@ def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, rati
             dt = 2 * 0.5 / dtc.item()
         if prec == np.float32:
             tuned = ctx.autotune(u0, reps=3)
-        sweep = GraphSweep(ctx, dt, interval, obja=1.0 / n_steps)
+        sweep = GraphSweep(ctx, dt, interval, obja=1.0 / n_steps, overlap=overlap)
         sweep.run(u0, interval)                    # capture + one segment
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         J, ua = sweep.run(u0, n_steps)
         t = time.perf_counter() - t1
         res[prec] = (J[0] / n_steps, ua.double().cpu().numpy(), t)
+        ovl = sweep.overlap
         print('[bench] cylinder %s: %.3f s, mean drag %.10e' % (np.dtype(prec).name, t, J[0] / n_steps),
               file=sys.stderr, flush=True)
     d32, g32, t32 = res[np.float32]
@@ -355,7 +358,7 @@ def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, rati
                                    'flux; %d steps primal + adjoint of the mean drag, checkpoints every %d steps, '
                                    'HIP-graph segments' % (n_theta, n_r, ratio, n_steps, interval),
                        'n_cells': nC, 'dt': dt},
-            'block_sizes': {k: v[0] for k, v in tuned.items()},
+            'block_sizes': {k: v[0] for k, v in tuned.items()}, 'overlapped_reverse': ovl,
             'value_fp64': nC * n_steps / t64, 'fp32_over_fp64': t64 / t32,
             'mean_drag_fp32': d32, 'mean_drag_fp64': d64, 'mean_drag_rel_err_fp32_vs_fp64': abs(d32 - d64) / abs(d64),
             'grad_rel_l2_fp32_vs_fp64': {'rho': rel(g32[0:1], g64[0:1]), 'rhoU': rel(g32[1:4], g64[1:4]),
@@ -382,6 +385,9 @@ def main():
     ap.add_argument('--cylinder', action='store_true',
                     help='instead: the 2D cylinder wake (512 x 256 O-grid, 5000 steps primal + adjoint of the mean '
                          'drag, checkpoints every 100, HIP-graph segments), FP32 and FP64')
+    ap.add_argument('--no-overlap', action='store_true',
+                    help='with --ensemble, --adjoint64, --cylinder: reverse segments in sequence (no re-run on a '
+                         'second stream alongside the adjoint steps)')
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a device-to-device copy of MIB MiB five times (counter calibration)')
     args = ap.parse_args()
@@ -393,14 +399,15 @@ def main():
     from tests import casefactory as cf
     assert torch.cuda.is_available(), 'bench.py needs the GPU'
     torch.cuda.set_device(0)
+    overlap = False if args.no_overlap else None
     if args.ensemble:
-        print(json.dumps(ensemble_bench(torch)))
+        print(json.dumps(ensemble_bench(torch, overlap=overlap)))
         return
     if args.cylinder:
-        print(json.dumps(cylinder_bench(torch)))
+        print(json.dumps(cylinder_bench(torch, overlap=overlap)))
         return
     if args.adjoint64:
-        print(json.dumps(adjoint64_bench(torch)))
+        print(json.dumps(adjoint64_bench(torch, overlap=overlap)))
         return
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
