@@ -282,7 +282,8 @@ def adjoint64_bench(torch, n=64, n_steps=400, interval=20, overlap=None):
             J, ua = sweep.run(u0, n_steps)
             t = time.perf_counter() - t1
             res[state, prec] = (float(J[0]), ua.double().cpu().numpy(), t)
-            ovl = sweep.overlap
+            if state == 'stage1' and prec == np.float32:
+                ovl = sweep.overlap
             print('[bench] TGV %d^3 adjoint, objective on %s, %s: %.3f s, J %.10e'
                   % (n, state, np.dtype(prec).name, t, J[0]), file=sys.stderr, flush=True)
             del sweep, ctx
@@ -343,7 +344,8 @@ def cylinder_bench(torch, n_steps=5000, interval=100, n_theta=512, n_r=256, rati
         J, ua = sweep.run(u0, n_steps)
         t = time.perf_counter() - t1
         res[prec] = (J[0] / n_steps, ua.double().cpu().numpy(), t)
-        ovl = sweep.overlap
+        if prec == np.float32:
+            ovl = sweep.overlap
         print('[bench] cylinder %s: %.3f s, mean drag %.10e' % (np.dtype(prec).name, t, J[0] / n_steps),
               file=sys.stderr, flush=True)
     d32, g32, t32 = res[np.float32]
